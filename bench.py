@@ -1,0 +1,153 @@
+#!/usr/bin/env python3
+"""Benchmark: NMF factorizations/s on one MI355X.
+
+Workload (the single-large-job configuration): V 20000 x 1000 FP32, planted rank 16
+(V = W*.H* (.) exp(N(0, 0.3^2)), W*, H* ~ Gamma(1,1)), k = 16, W0/H0 ~ U(0,1] from Philox,
+KL multiplicative updates for a fixed 500 iterations with the convergence test (clamp +
+classification + connectivity difference) every 10 iterations.  One step = one whole
+factorization (initialisation + 500 iterations) with V already resident in HBM.
+
+Prints one JSON line with the throughput, the HBM roofline of the dominant kernel (HIP
+events on the job's stream) and a CPU baseline (the oracle's scalar C rendering of the
+reference updates on the host cores, on a bounded sample of iterations).
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline]
+"""
+from __future__ import annotations
+
+import argparse
+import importlib.util
+import json
+import os
+import pathlib
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = pathlib.Path(__file__).resolve().parent
+HBM_PEAK_GBS = 8000.0
+
+N, M, K, KSTAR, SIGMA, ITERS, T = 20000, 1000, 16, 16, 0.3, 500, 10
+SEED_V, SEED_INIT = 20250101, 7
+
+
+def load_package():
+    spec = importlib.util.spec_from_file_location("bionmf_gpu_amd", ROOT / "bionmf-gpu_amd" / "__init__.py")
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["bionmf_gpu_amd"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def cpu_baseline(sample_iters: int) -> dict:
+    """Oracle's FP32 scalar rendering of the reference update sequence on the host cores
+    (TEST INFRASTRUCTURE, timed as a reported baseline only)."""
+    exe = ROOT / "oracle" / "nmf_cpu_baseline"
+    if not exe.exists():
+        return None
+    threads = min(16, os.cpu_count() or 1)
+    cmd = [str(exe), "--n", str(N), "--m", str(M), "--k", str(K), "--iters", str(sample_iters),
+           "--T", str(T), "--S", "1000000000", "--gen", "planted", "--kstar", str(KSTAR),
+           "--sigma", str(SIGMA), "--threads", str(threads)]
+    out = subprocess.run(cmd, check=True, capture_output=True, text=True, timeout=600).stdout
+    r = json.loads(out.strip().splitlines()[-1])
+    s_iter = r["s_per_iter"]
+    return {"value": 1.0 / (ITERS * s_iter), "unit": "factorizations/s", "cores": r["threads"],
+            "kind": "port",
+            "sample": f"{sample_iters} iterations of the {N}x{M} k={K} FP32 update loop "
+                      f"(s/iter {s_iter:.4f}) extrapolated to {ITERS} iterations"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-iters", type=int, default=30)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; the multi-GPU row-block "
+                                   "split is a design (see DESIGN.md), --gpus must be 1"}))
+        sys.exit(2)
+
+    pkg = load_package()
+    pkg.load()
+    if pkg.gpu_count() < 1:
+        print(json.dumps({"error": "no HIP device visible"}))
+        sys.exit(1)
+
+    job = pkg.Job(N, M, K)
+    job.gen_V_planted(SEED_V, KSTAR, SIGMA)
+
+    def step(s):
+        job.init_random(SEED_INIT + s)
+        t0 = time.perf_counter()
+        r = job.run(ITERS, T, 1_000_000_000)
+        return time.perf_counter() - t0, r
+
+    for s in range(args.warmup):
+        step(s)
+    job.sync()
+    lat = []
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        dt, r = step(1000 + s)
+        lat.append(dt)
+    job.sync()
+    elapsed = time.perf_counter() - t0
+    assert r["iterations"] == ITERS
+
+    dv, dvwh, kl = job.dot_kl()
+    value = args.steps / elapsed
+
+    # Dominant-kernel roofline: average launch duration from HIP events on the job stream.
+    tm = job.time_iterations(50)
+    bytes_partial = 4.0 * (N * M + N * K + M * K)   # V once + both factors (algorithmic reads)
+    kern = "partial_W" if tm["ms_partial_W"] >= tm["ms_partial_H"] else "partial_H"
+    ms = tm["ms_" + kern]
+    achieved = bytes_partial / (ms * 1e-3) / 1e9
+    iter_bytes = 4.0 * (2 * N * M + 3 * N * K + 3 * M * K)
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        try:
+            cpu = cpu_baseline(args.cpu_iters)
+        except Exception as e:  # reported, never fatal for the GPU number
+            cpu = {"error": str(e)}
+
+    line = {
+        "metric": "NMF factorizations/sec at <=1e-3 rel KL err; p50/p99 job latency; % HBM roofline",
+        "value": value,
+        "unit": "factorizations/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (planted rank-16 V generated on the device from a Philox seed)",
+        "config": {"workload": f"single large job: V {N}x{M} planted rank {KSTAR} (Gamma(1,1) factors, "
+                               f"lognormal sigma={SIGMA} noise), k={K}, {ITERS} iterations, test every {T}",
+                   "n": N, "m": M, "k": K, "iterations": ITERS, "niter_test_conv": T},
+        "latency_ms": {"p50": 1e3 * float(np.percentile(lat, 50)), "p99": 1e3 * float(np.percentile(lat, 99))},
+        "ms_per_iteration": tm["ms_per_iter"],
+        "kernels_ms": {k_: v for k_, v in tm.items() if k_ != "ms_per_iter"},
+        "iteration_GBps": iter_bytes / (tm["ms_per_iter"] * 1e-3) / 1e9,
+        "kl_final": kl,
+        "frobenius_ratio": float(np.sqrt(dvwh / dv)),
+        "roofline": {"bound": "hbm", "kernel": kern, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_launch": bytes_partial, "ms_per_launch": ms},
+        "cpu_baseline": cpu,
+    }
+    job.close()
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,245 @@
+"""bionmf-gpu_amd -- Python mirror of the MI355X KL-NMF C-ABI (libnmfgpu_amd.so).
+
+The product is the C library: C host code (src/NMF_GPU.c-style driver, the
+NMF_routines.h / matrix_operations.h / GPU_setup.h entry points and a job-handle API)
+over hand-written gfx950 HIP kernels.  This module only binds it with ctypes so that
+tests and the benchmark can drive it with numpy arrays; it never computes anything
+itself and raises if the library is missing.
+
+The package directory name contains a hyphen, so it is loaded by path:
+
+    import importlib.util, pathlib
+    spec = importlib.util.spec_from_file_location(
+        "bionmf_gpu_amd", pathlib.Path("bionmf-gpu_amd/__init__.py"))
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import pathlib
+
+import numpy as np
+
+_HERE = pathlib.Path(__file__).resolve().parent
+LIB_PATH = _HERE / "libnmfgpu_amd.so"
+DRIVER_PATH = _HERE / "NMF_GPU"
+INCLUDE_DIR = _HERE.parent / "include"
+
+_lib = None
+
+
+class NmfkError(RuntimeError):
+    pass
+
+
+class Options(C.Structure):
+    """nmfk_options (include/nmfk.h): the reference's -k, -i, -j, -t flags."""
+
+    _fields_ = [("k", C.c_uint32), ("nIters", C.c_uint32),
+                ("niter_test_conv", C.c_uint32), ("stop_threshold", C.c_uint32)]
+
+
+class Result(C.Structure):
+    _fields_ = [("iterations", C.c_uint32), ("converged", C.c_uint32),
+                ("ntests", C.c_uint32), ("pad_", C.c_uint32),
+                ("last_diff", C.c_uint64), ("seconds", C.c_double)]
+
+    def as_dict(self):
+        return {"iterations": self.iterations, "converged": bool(self.converged),
+                "ntests": self.ntests, "last_diff": self.last_diff,
+                "seconds": self.seconds}
+
+
+class Timing(C.Structure):
+    _fields_ = [("ms_per_iter", C.c_double), ("ms_partial_H", C.c_double),
+                ("ms_finish_H", C.c_double), ("ms_partial_W", C.c_double),
+                ("ms_finish_W", C.c_double)]
+
+
+class JobInfo(C.Structure):
+    _fields_ = [("n", C.c_uint32), ("m", C.c_uint32), ("k", C.c_uint32), ("kt", C.c_uint32),
+                ("ldv", C.c_size_t), ("ldvt", C.c_size_t),
+                ("dV", C.c_void_p), ("dVt", C.c_void_p), ("dW", C.c_void_p), ("dH", C.c_void_p),
+                ("stream", C.c_void_p)]
+
+
+_F32P = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_U32P = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+
+
+def _sig(lib, name, res, args):
+    f = getattr(lib, name)
+    f.restype = res
+    f.argtypes = args
+    return f
+
+
+def load(path: os.PathLike | str | None = None):
+    """Loads libnmfgpu_amd.so (built by __graft_entry__.build() or `make`)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    p = pathlib.Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise NmfkError(f"{p} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()')")
+    lib = C.CDLL(str(p))
+    vp = C.c_void_p
+    u32, u64, sz, dbl, i32 = C.c_uint32, C.c_uint64, C.c_size_t, C.c_double, C.c_int
+    _sig(lib, "nmfk_last_error", C.c_char_p, [])
+    _sig(lib, "nmfk_set_device", i32, [i32])
+    _sig(lib, "nmfk_default_options", None, [C.POINTER(Options)])
+    _sig(lib, "nmfk_job_create", vp, [u32, u32, u32])
+    _sig(lib, "nmfk_job_destroy", None, [vp])
+    _sig(lib, "nmfk_job_bytes", sz, [u32, u32, u32])
+    _sig(lib, "nmfk_job_set_V", i32, [vp, _F32P, sz])
+    _sig(lib, "nmfk_job_set_V_device", i32, [vp, vp, sz])
+    _sig(lib, "nmfk_job_gen_V_lognormal", i32, [vp, u64, dbl, dbl])
+    _sig(lib, "nmfk_job_gen_V_planted", i32, [vp, u64, u32, dbl])
+    _sig(lib, "nmfk_job_get_V", i32, [vp, _F32P, sz])
+    _sig(lib, "nmfk_job_init_random", i32, [vp, u64])
+    _sig(lib, "nmfk_job_set_WH", i32, [vp, _F32P, sz, _F32P, sz])
+    _sig(lib, "nmfk_job_get_W", i32, [vp, _F32P, sz])
+    _sig(lib, "nmfk_job_get_H", i32, [vp, _F32P, sz])
+    _sig(lib, "nmfk_job_run", i32, [vp, C.POINTER(Options), C.POINTER(Result)])
+    _sig(lib, "nmfk_job_update_H", i32, [vp])
+    _sig(lib, "nmfk_job_update_W", i32, [vp])
+    _sig(lib, "nmfk_job_adjust", i32, [vp])
+    _sig(lib, "nmfk_job_get_classification", i32, [vp, _U32P])
+    _sig(lib, "nmfk_job_dot_kl", i32, [vp, C.POINTER(dbl), C.POINTER(dbl), C.POINTER(dbl)])
+    _sig(lib, "nmfk_job_sync", i32, [vp])
+    _sig(lib, "nmfk_job_info_get", i32, [vp, C.POINTER(JobInfo)])
+    _sig(lib, "nmfk_job_time_iterations", i32, [vp, u32, C.POINTER(Timing)])
+    _lib = lib
+    return lib
+
+
+def last_error() -> str:
+    return load().nmfk_last_error().decode(errors="replace")
+
+
+def _check(rc: int, what: str):
+    if rc != 0:
+        raise NmfkError(f"{what} failed ({rc}): {last_error()}")
+
+
+def gpu_count() -> int:
+    """Number of HIP devices, from the HIP runtime directly (no torch)."""
+    try:
+        hip = C.CDLL("libamdhip64.so")
+    except OSError:
+        try:
+            hip = C.CDLL("/opt/rocm/lib/libamdhip64.so")
+        except OSError:
+            return 0
+    n = C.c_int(0)
+    rc = hip.hipGetDeviceCount(C.byref(n))
+    return n.value if rc == 0 else 0
+
+
+def default_options(k: int = 2, nIters: int = 2000, niter_test_conv: int = 10,
+                    stop_threshold: int = 40) -> Options:
+    return Options(k, nIters, niter_test_conv, stop_threshold)
+
+
+class Job:
+    """One factorization V (n x m) ~ W (n x k) . H^T (H is m x k), resident on the GPU."""
+
+    def __init__(self, n: int, m: int, k: int):
+        self.lib = load()
+        self.n, self.m, self.k = int(n), int(m), int(k)
+        self.h = self.lib.nmfk_job_create(self.n, self.m, self.k)
+        if not self.h:
+            raise NmfkError(f"nmfk_job_create failed: {last_error()}")
+
+    def close(self):
+        if self.h:
+            self.lib.nmfk_job_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # inputs -------------------------------------------------------------
+    def set_V(self, V: np.ndarray):
+        V = np.ascontiguousarray(V, dtype=np.float32)
+        assert V.shape == (self.n, self.m), V.shape
+        _check(self.lib.nmfk_job_set_V(self.h, V, self.m), "set_V")
+
+    def gen_V_lognormal(self, seed: int, mu: float, sigma: float):
+        _check(self.lib.nmfk_job_gen_V_lognormal(self.h, seed, mu, sigma), "gen_V_lognormal")
+
+    def gen_V_planted(self, seed: int, kstar: int, sigma: float):
+        _check(self.lib.nmfk_job_gen_V_planted(self.h, seed, kstar, sigma), "gen_V_planted")
+
+    def get_V(self) -> np.ndarray:
+        out = np.empty((self.n, self.m), np.float32)
+        _check(self.lib.nmfk_job_get_V(self.h, out, self.m), "get_V")
+        return out
+
+    def init_random(self, seed: int):
+        _check(self.lib.nmfk_job_init_random(self.h, seed), "init_random")
+
+    def set_WH(self, W: np.ndarray, H: np.ndarray):
+        W = np.ascontiguousarray(W, dtype=np.float32)
+        H = np.ascontiguousarray(H, dtype=np.float32)
+        assert W.shape == (self.n, self.k) and H.shape == (self.m, self.k)
+        _check(self.lib.nmfk_job_set_WH(self.h, W, self.k, H, self.k), "set_WH")
+
+    # outputs ------------------------------------------------------------
+    def get_W(self) -> np.ndarray:
+        out = np.empty((self.n, self.k), np.float32)
+        _check(self.lib.nmfk_job_get_W(self.h, out, self.k), "get_W")
+        return out
+
+    def get_H(self) -> np.ndarray:
+        out = np.empty((self.m, self.k), np.float32)
+        _check(self.lib.nmfk_job_get_H(self.h, out, self.k), "get_H")
+        return out
+
+    def get_classification(self) -> np.ndarray:
+        out = np.empty(self.m, np.uint32)
+        _check(self.lib.nmfk_job_get_classification(self.h, out), "get_classification")
+        return out
+
+    def dot_kl(self):
+        a, b, c = C.c_double(), C.c_double(), C.c_double()
+        _check(self.lib.nmfk_job_dot_kl(self.h, C.byref(a), C.byref(b), C.byref(c)), "dot_kl")
+        return a.value, b.value, c.value
+
+    # the loop -----------------------------------------------------------
+    def run(self, nIters=2000, niter_test_conv=10, stop_threshold=40) -> dict:
+        opt = Options(self.k, nIters, niter_test_conv, stop_threshold)
+        res = Result()
+        _check(self.lib.nmfk_job_run(self.h, C.byref(opt), C.byref(res)), "run")
+        return res.as_dict()
+
+    def update_H(self):
+        _check(self.lib.nmfk_job_update_H(self.h), "update_H")
+
+    def update_W(self):
+        _check(self.lib.nmfk_job_update_W(self.h), "update_W")
+
+    def adjust(self):
+        _check(self.lib.nmfk_job_adjust(self.h), "adjust")
+
+    def sync(self):
+        _check(self.lib.nmfk_job_sync(self.h), "sync")
+
+    def info(self) -> JobInfo:
+        inf = JobInfo()
+        _check(self.lib.nmfk_job_info_get(self.h, C.byref(inf)), "info")
+        return inf
+
+    def time_iterations(self, iters: int) -> dict:
+        t = Timing()
+        _check(self.lib.nmfk_job_time_iterations(self.h, iters, C.byref(t)), "time_iterations")
+        return {f: getattr(t, f) for f, _ in Timing._fields_}

@@ -1,0 +1,609 @@
+/*
+ * nmfk_core.c -- host side (C99) of one KL-NMF factorization on one MI355X.
+ *
+ * Replaces the buffer set-up of src/GPU_setup.c:1289-1610 and the loop of
+ * src/NMF_GPU.c:312-924 with a job handle.  One iteration is four launches on the job's
+ * stream (H half: partial + finish, W half: partial + finish); every niter_test_conv
+ * iterations the test kernel clamps, classifies, counts the connectivity difference and
+ * applies the stop rule on the device.  A whole test interval is captured once as a
+ * hipGraph and replayed; the host reads the 32-byte state of interval c while interval
+ * c+1 is already queued, and every kernel returns at once after the state says
+ * "converged", so at most one no-op interval is ever queued past convergence.
+ */
+#include "nmfk_internal.h"
+
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+static __thread char g_err[512];
+
+void nmfk_set_error(const char *fmt, ...)
+{
+	va_list ap;
+	va_start(ap, fmt);
+	vsnprintf(g_err, sizeof g_err, fmt, ap);
+	va_end(ap);
+}
+
+const char *nmfk_last_error(void)
+{
+	return g_err;
+}
+
+int nmfk_hip_fail(int err, const char *what)
+{
+	if (err == 0)
+		return 0;
+	nmfk_set_error("%s: %s", what, hipGetErrorString((hipError_t)err));
+	return NMFK_EHIP;
+}
+
+double nmfk_now(void)
+{
+	struct timespec ts;
+	clock_gettime(CLOCK_MONOTONIC, &ts);
+	return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
+void nmfk_default_options(nmfk_options *opt)
+{
+	opt->k = 2;
+	opt->nIters = 2000;
+	opt->niter_test_conv = 10;
+	opt->stop_threshold = 40;
+}
+
+int nmfk_set_device(int dev)
+{
+	return nmfk_hip_fail(hipSetDevice(dev), "hipSetDevice");
+}
+
+static size_t round_up(size_t x, size_t a)
+{
+	return (x + a - 1) / a * a;
+}
+
+/* Sizes of every device buffer of a job, shared by create() and bytes(). */
+typedef struct job_layout {
+	uint32_t kt;
+	size_t ldv, ldvt;
+	nmfk_hplan planH, planW;
+	size_t n_V, n_Vt, n_W, n_H, n_part, n_colW, n_colH, n_dot;
+} job_layout;
+
+static void layout_of(uint32_t n, uint32_t m, uint32_t k, job_layout *L)
+{
+	L->kt = nmfk_kt(k);
+	L->ldv = round_up(m, 4);
+	L->ldvt = round_up(n, 4);
+	nmfk_plan_half(m, n, L->kt, &L->planH);
+	nmfk_plan_half(n, m, L->kt, &L->planW);
+	L->n_V = (size_t)n * L->ldv;
+	L->n_Vt = (size_t)m * L->ldvt;
+	L->n_W = (size_t)n * L->kt;
+	L->n_H = (size_t)m * L->kt;
+	size_t pH = (size_t)L->planH.S * m * L->kt, pW = (size_t)L->planW.S * n * L->kt;
+	L->n_part = pH > pW ? pH : pW;
+	L->n_colW = (size_t)((n + NMFK_FIN_ROWS - 1) / NMFK_FIN_ROWS) * L->kt;
+	L->n_colH = (size_t)((m + NMFK_FIN_ROWS - 1) / NMFK_FIN_ROWS) * L->kt;
+	L->n_dot = (size_t)nmfk_dot_kl_blocks(n, m) * 3;
+}
+
+size_t nmfk_job_bytes(uint32_t n, uint32_t m, uint32_t k)
+{
+	job_layout L;
+	layout_of(n, m, k, &L);
+	size_t b = 4 * (L.n_V + L.n_Vt + L.n_W + L.n_H + L.n_part + L.n_colW + L.n_colH);
+	b += 8 * L.n_dot + 4 * 2 * (size_t)m + sizeof(nmfk_state) + 7 * 256;
+	return b;
+}
+
+static int dalloc(void **p, size_t bytes, const char *what)
+{
+	*p = NULL;
+	if (bytes == 0)
+		bytes = 4;
+	return nmfk_hip_fail(hipMalloc(p, bytes), what);
+}
+
+nmfk_job *nmfk_job_create(uint32_t n, uint32_t m, uint32_t k)
+{
+	if (n < 1 || m < 1 || k < 1 || k > NMFK_KMAX) {
+		nmfk_set_error("nmfk_job_create: invalid shape n=%u m=%u k=%u (1 <= k <= %d)", n, m, k, NMFK_KMAX);
+		return NULL;
+	}
+	if (k > n || k > m) {
+		/* src/NMF_GPU.c:1130 */
+		nmfk_set_error("nmfk_job_create: k=%u must be <= min(n=%u, m=%u)", k, n, m);
+		return NULL;
+	}
+	nmfk_job *j = (nmfk_job *)calloc(1, sizeof(nmfk_job));
+	if (!j) {
+		nmfk_set_error("nmfk_job_create: out of host memory");
+		return NULL;
+	}
+	job_layout L;
+	layout_of(n, m, k, &L);
+	j->n = n, j->m = m, j->k = k, j->kt = L.kt;
+	j->ldv = L.ldv, j->ldvt = L.ldvt;
+	j->planH = L.planH, j->planW = L.planW;
+	j->ncolW = (n + NMFK_FIN_ROWS - 1) / NMFK_FIN_ROWS;
+	j->ncolH = (m + NMFK_FIN_ROWS - 1) / NMFK_FIN_ROWS;
+	j->ndot = nmfk_dot_kl_blocks(n, m);
+	int e = 0;
+	e |= dalloc((void **)&j->V, 4 * L.n_V, "hipMalloc(V)");
+	if (!e) e |= dalloc((void **)&j->Vt, 4 * L.n_Vt, "hipMalloc(Vt)");
+	if (!e) e |= dalloc((void **)&j->W, 4 * L.n_W, "hipMalloc(W)");
+	if (!e) e |= dalloc((void **)&j->H, 4 * L.n_H, "hipMalloc(H)");
+	if (!e) e |= dalloc((void **)&j->part, 4 * L.n_part, "hipMalloc(partials)");
+	if (!e) e |= dalloc((void **)&j->colW, 4 * L.n_colW, "hipMalloc(colW)");
+	if (!e) e |= dalloc((void **)&j->colH, 4 * L.n_colH, "hipMalloc(colH)");
+	if (!e) e |= dalloc((void **)&j->dot, 8 * L.n_dot, "hipMalloc(dot)");
+	if (!e) e |= dalloc((void **)&j->cls, 4 * (size_t)m, "hipMalloc(cls)");
+	if (!e) e |= dalloc((void **)&j->last, 4 * (size_t)m, "hipMalloc(last)");
+	if (!e) e |= dalloc((void **)&j->st, sizeof(nmfk_state), "hipMalloc(state)");
+	if (!e) e |= nmfk_hip_fail(hipHostMalloc((void **)&j->h_st, 2 * sizeof(nmfk_state), hipHostMallocDefault),
+				   "hipHostMalloc(state)");
+	if (!e) e |= nmfk_hip_fail(hipStreamCreateWithFlags(&j->stream, hipStreamNonBlocking), "hipStreamCreate");
+	if (!e) e |= nmfk_hip_fail(hipEventCreateWithFlags(&j->ev[0], hipEventDisableTiming), "hipEventCreate");
+	if (!e) e |= nmfk_hip_fail(hipEventCreateWithFlags(&j->ev[1], hipEventDisableTiming), "hipEventCreate");
+	if (e) {
+		char msg[512];
+		snprintf(msg, sizeof msg, "%s", g_err);
+		nmfk_job_destroy(j);
+		nmfk_set_error("%s", msg);
+		return NULL;
+	}
+	/* Zero W and H so their padding columns start (and stay) at zero. */
+	hipMemsetAsync(j->W, 0, 4 * L.n_W, j->stream);
+	hipMemsetAsync(j->H, 0, 4 * L.n_H, j->stream);
+	hipMemsetAsync(j->V, 0, 4 * L.n_V, j->stream);
+	hipMemsetAsync(j->Vt, 0, 4 * L.n_Vt, j->stream);
+	if (nmfk_hip_fail(hipStreamSynchronize(j->stream), "job create")) {
+		nmfk_job_destroy(j);
+		return NULL;
+	}
+	return j;
+}
+
+void nmfk_job_destroy(nmfk_job *j)
+{
+	if (!j)
+		return;
+	if (j->stream)
+		hipStreamSynchronize(j->stream);
+	if (j->chunk_exec)
+		hipGraphExecDestroy(j->chunk_exec);
+	hipFree(j->V), hipFree(j->Vt), hipFree(j->W), hipFree(j->H), hipFree(j->part);
+	hipFree(j->colW), hipFree(j->colH), hipFree(j->dot), hipFree(j->cls), hipFree(j->last);
+	hipFree(j->st);
+	if (j->h_st)
+		hipHostFree(j->h_st);
+	if (j->ev[0])
+		hipEventDestroy(j->ev[0]);
+	if (j->ev[1])
+		hipEventDestroy(j->ev[1]);
+	if (j->stream)
+		hipStreamDestroy(j->stream);
+	free(j);
+}
+
+int nmfk_job_info_get(const nmfk_job *j, nmfk_job_info *info)
+{
+	info->n = j->n, info->m = j->m, info->k = j->k, info->kt = j->kt;
+	info->ldv = j->ldv, info->ldvt = j->ldvt;
+	info->dV = j->V, info->dVt = j->Vt, info->dW = j->W, info->dH = j->H;
+	info->stream = (void *)j->stream;
+	return 0;
+}
+
+int nmfk_job_sync(nmfk_job *j)
+{
+	return nmfk_hip_fail(hipStreamSynchronize(j->stream), "hipStreamSynchronize");
+}
+
+/* ---------------------------------------------------------------- inputs */
+
+static int finish_V(nmfk_job *j)
+{
+	j->colW_valid = 0;
+	int e = nmfk_launch_transpose(j->V, j->n, j->m, j->ldv, j->Vt, j->ldvt, j->stream);
+	if (e)
+		return nmfk_hip_fail(e, "transpose V");
+	return nmfk_hip_fail(hipStreamSynchronize(j->stream), "set V");
+}
+
+int nmfk_job_set_V(nmfk_job *j, const float *V, size_t ldv)
+{
+	if (ldv < j->m) {
+		nmfk_set_error("nmfk_job_set_V: ldv=%zu < m=%u", ldv, j->m);
+		return NMFK_EINVAL;
+	}
+	int e = hipMemcpy2DAsync(j->V, j->ldv * 4, V, ldv * 4, (size_t)j->m * 4, j->n, hipMemcpyHostToDevice,
+				 j->stream);
+	if (e)
+		return nmfk_hip_fail(e, "upload V");
+	return finish_V(j);
+}
+
+int nmfk_job_set_V_device(nmfk_job *j, const float *dV, size_t ldv)
+{
+	if (ldv < j->m) {
+		nmfk_set_error("nmfk_job_set_V_device: ldv=%zu < m=%u", ldv, j->m);
+		return NMFK_EINVAL;
+	}
+	int e = hipMemcpy2DAsync(j->V, j->ldv * 4, dV, ldv * 4, (size_t)j->m * 4, j->n, hipMemcpyDeviceToDevice,
+				 j->stream);
+	if (e)
+		return nmfk_hip_fail(e, "copy V");
+	return finish_V(j);
+}
+
+int nmfk_job_gen_V_lognormal(nmfk_job *j, uint64_t seed, double mu, double sigma)
+{
+	int e = nmfk_launch_gen_lognormal(j->V, j->n, j->m, (uint32_t)j->ldv, seed, mu, sigma, j->stream);
+	if (e)
+		return nmfk_hip_fail(e, "gen lognormal");
+	return finish_V(j);
+}
+
+int nmfk_job_gen_V_planted(nmfk_job *j, uint64_t seed, uint32_t kstar, double sigma)
+{
+	double *scratch = NULL;
+	int e = dalloc((void **)&scratch, 8 * (size_t)kstar * ((size_t)j->n + j->m), "hipMalloc(planted)");
+	if (e)
+		return e;
+	e = nmfk_launch_gen_planted(j->V, j->n, j->m, (uint32_t)j->ldv, kstar, seed, sigma, scratch, j->stream);
+	if (e) {
+		hipFree(scratch);
+		return nmfk_hip_fail(e, "gen planted");
+	}
+	e = finish_V(j);
+	hipFree(scratch);
+	return e;
+}
+
+int nmfk_job_get_V(nmfk_job *j, float *V, size_t ldv)
+{
+	if (ldv < j->m) {
+		nmfk_set_error("nmfk_job_get_V: ldv < m");
+		return NMFK_EINVAL;
+	}
+	int e = hipMemcpy2DAsync(V, ldv * 4, j->V, j->ldv * 4, (size_t)j->m * 4, j->n, hipMemcpyDeviceToHost,
+				 j->stream);
+	if (e)
+		return nmfk_hip_fail(e, "download V");
+	return nmfk_job_sync(j);
+}
+
+int nmfk_job_init_random(nmfk_job *j, uint64_t seed)
+{
+	/* W then H from the same seed, as set_random_values() draws d_W then d_H
+	 * (src/NMF_GPU.c:356-430). */
+	int e = nmfk_launch_uniform(j->W, j->n, j->k, j->kt, seed, 0, j->stream);
+	if (!e)
+		e = nmfk_launch_uniform(j->H, j->m, j->k, j->kt, seed, 1, j->stream);
+	j->colW_valid = 0;
+	if (e)
+		return nmfk_hip_fail(e, "init random");
+	return nmfk_job_sync(j);
+}
+
+static int put_factor(nmfk_job *j, float *d, uint32_t rows, const float *h, size_t ld)
+{
+	if (ld < j->k) {
+		nmfk_set_error("leading dimension %zu < k=%u", ld, j->k);
+		return NMFK_EINVAL;
+	}
+	int e = hipMemsetAsync(d, 0, (size_t)rows * j->kt * 4, j->stream);
+	if (!e)
+		e = hipMemcpy2DAsync(d, (size_t)j->kt * 4, h, ld * 4, (size_t)j->k * 4, rows, hipMemcpyHostToDevice,
+				     j->stream);
+	return nmfk_hip_fail(e, "upload factor");
+}
+
+int nmfk_job_set_WH(nmfk_job *j, const float *W, size_t ldw, const float *H, size_t ldh)
+{
+	int e = put_factor(j, j->W, j->n, W, ldw);
+	if (!e)
+		e = put_factor(j, j->H, j->m, H, ldh);
+	j->colW_valid = 0;
+	if (e)
+		return e;
+	return nmfk_job_sync(j);
+}
+
+static int get_factor(nmfk_job *j, const float *d, uint32_t rows, float *h, size_t ld)
+{
+	if (ld < j->k) {
+		nmfk_set_error("leading dimension %zu < k=%u", ld, j->k);
+		return NMFK_EINVAL;
+	}
+	int e = hipMemcpy2DAsync(h, ld * 4, d, (size_t)j->kt * 4, (size_t)j->k * 4, rows, hipMemcpyDeviceToHost,
+				 j->stream);
+	if (e)
+		return nmfk_hip_fail(e, "download factor");
+	return nmfk_job_sync(j);
+}
+
+int nmfk_job_get_W(nmfk_job *j, float *W, size_t ldw)
+{
+	return get_factor(j, j->W, j->n, W, ldw);
+}
+
+int nmfk_job_get_H(nmfk_job *j, float *H, size_t ldh)
+{
+	return get_factor(j, j->H, j->m, H, ldh);
+}
+
+/* ---------------------------------------------------------------- half-steps */
+
+int nmfk_enqueue_half_H(nmfk_job *j, const nmfk_state *st)
+{
+	int e = nmfk_launch_partial(j->V, j->ldv, j->H, j->W, j->m, j->n, j->kt, &j->planH, j->part, st,
+				    j->stream);
+	if (!e)
+		e = nmfk_launch_finish(j->part, j->planH.S, j->H, j->m, j->k, j->kt, j->colW, j->ncolW, j->colH, 0,
+				       st, j->stream);
+	return e;
+}
+
+int nmfk_enqueue_half_W(nmfk_job *j, const nmfk_state *st, int clamp)
+{
+	int e = nmfk_launch_partial(j->Vt, j->ldvt, j->W, j->H, j->n, j->m, j->kt, &j->planW, j->part, st,
+				    j->stream);
+	if (!e)
+		e = nmfk_launch_finish(j->part, j->planW.S, j->W, j->n, j->k, j->kt, j->colH, j->ncolH, j->colW,
+				       clamp, st, j->stream);
+	return e;
+}
+
+/* update_H(): accum_W = colsum(W) (src/NMF_routines.c:720), then the H half-step. */
+int nmfk_job_update_H(nmfk_job *j)
+{
+	int e = nmfk_launch_colsum_parts(j->W, j->n, j->kt, j->colW, j->stream);
+	if (!e)
+		e = nmfk_enqueue_half_H(j, NULL);
+	return nmfk_hip_fail(e, "update_H");
+}
+
+/* update_W(): accum_H = colsum(H) (src/NMF_routines.c:1360), then the W half-step. */
+int nmfk_job_update_W(nmfk_job *j)
+{
+	int e = nmfk_launch_colsum_parts(j->H, j->m, j->kt, j->colH, j->stream);
+	if (!e)
+		e = nmfk_enqueue_half_W(j, NULL, 0);
+	j->colW_valid = 1;
+	return nmfk_hip_fail(e, "update_W");
+}
+
+int nmfk_job_adjust(nmfk_job *j)
+{
+	int e = nmfk_launch_adjust(j->H, j->m, j->k, j->kt, j->stream);
+	if (!e)
+		e = nmfk_launch_adjust(j->W, j->n, j->k, j->kt, j->stream);
+	j->colW_valid = 0;
+	return nmfk_hip_fail(e, "adjust");
+}
+
+int nmfk_job_get_classification(nmfk_job *j, uint32_t *cls)
+{
+	int e = nmfk_launch_idx_max(j->H, j->m, j->k, j->kt, j->cls, j->stream);
+	if (!e)
+		e = hipMemcpyAsync(cls, j->cls, 4 * (size_t)j->m, hipMemcpyDeviceToHost, j->stream);
+	if (e)
+		return nmfk_hip_fail(e, "classification");
+	return nmfk_job_sync(j);
+}
+
+/* ---------------------------------------------------------------- the loop */
+
+static int enqueue_interval(nmfk_job *j, uint32_t T)
+{
+	int e = 0;
+	for (uint32_t i = 0; i < T && !e; i++) {
+		e = nmfk_enqueue_half_H(j, j->st);
+		if (!e)
+			e = nmfk_enqueue_half_W(j, j->st, i + 1 == T);
+	}
+	if (!e)
+		e = nmfk_launch_test(j->H, j->m, j->k, j->kt, j->cls, j->last, j->st, j->stream);
+	return e;
+}
+
+static int build_interval_graph(nmfk_job *j, uint32_t T)
+{
+	if (j->chunk_exec && j->chunk_T == T)
+		return 0;
+	if (j->chunk_exec) {
+		hipGraphExecDestroy(j->chunk_exec);
+		j->chunk_exec = NULL;
+	}
+	hipGraph_t g = NULL;
+	int e = hipStreamBeginCapture(j->stream, hipStreamCaptureModeThreadLocal);
+	if (e)
+		return nmfk_hip_fail(e, "hipStreamBeginCapture");
+	int e2 = enqueue_interval(j, T);
+	e = hipStreamEndCapture(j->stream, &g);
+	if (e2)
+		return nmfk_hip_fail(e2, "capture interval");
+	if (e)
+		return nmfk_hip_fail(e, "hipStreamEndCapture");
+	e = hipGraphInstantiate(&j->chunk_exec, g, NULL, NULL, 0);
+	hipGraphDestroy(g);
+	if (e)
+		return nmfk_hip_fail(e, "hipGraphInstantiate");
+	j->chunk_T = T;
+	return 0;
+}
+
+int nmfk_job_run(nmfk_job *j, const nmfk_options *opt, nmfk_result *res)
+{
+	if (!opt || opt->k != j->k || opt->niter_test_conv == 0) {
+		nmfk_set_error("nmfk_job_run: options do not match the job (k=%u, niter_test_conv>0)", j->k);
+		return NMFK_EINVAL;
+	}
+	const uint32_t T = opt->niter_test_conv;
+	const uint32_t niter_conv = opt->nIters / T, niter_rem = opt->nIters % T;
+	int e = nmfk_launch_state_reset(j->st, opt->stop_threshold, j->last, j->m, j->stream);
+	if (!e)
+		e = nmfk_launch_colsum_parts(j->W, j->n, j->kt, j->colW, j->stream);
+	if (e)
+		return nmfk_hip_fail(e, "run setup");
+	if (niter_conv > 0 && (e = build_interval_graph(j, T)) != 0)
+		return e;
+	e = nmfk_hip_fail(hipStreamSynchronize(j->stream), "run setup");
+	if (e)
+		return e;
+
+	const double t0 = nmfk_now();
+	int converged = 0;
+	for (uint32_t c = 0; c < niter_conv; c++) {
+		e = hipGraphLaunch(j->chunk_exec, j->stream);
+		if (!e)
+			e = hipMemcpyAsync(&j->h_st[c & 1], j->st, sizeof(nmfk_state), hipMemcpyDeviceToHost, j->stream);
+		if (!e)
+			e = hipEventRecord(j->ev[c & 1], j->stream);
+		if (e)
+			return nmfk_hip_fail(e, "interval launch");
+		if (c >= 1) {
+			e = hipEventSynchronize(j->ev[(c - 1) & 1]);
+			if (e)
+				return nmfk_hip_fail(e, "interval wait");
+			if (j->h_st[(c - 1) & 1].converged) {
+				converged = 1;
+				break;
+			}
+		}
+	}
+	if (!converged && niter_rem) {
+		for (uint32_t i = 0; i < niter_rem && !e; i++) {
+			e = nmfk_enqueue_half_H(j, j->st);
+			if (!e)
+				e = nmfk_enqueue_half_W(j, j->st, 0);
+		}
+		if (e)
+			return nmfk_hip_fail(e, "remaining iterations");
+	}
+	e = hipMemcpyAsync(&j->h_st[0], j->st, sizeof(nmfk_state), hipMemcpyDeviceToHost, j->stream);
+	if (!e)
+		e = hipStreamSynchronize(j->stream);
+	if (e)
+		return nmfk_hip_fail(e, "run");
+	const double t1 = nmfk_now();
+	const nmfk_state s = j->h_st[0];
+	j->colW_valid = 1;
+	if (res) {
+		res->converged = s.converged;
+		res->ntests = s.ntests;
+		res->last_diff = s.last_diff;
+		res->iterations = s.converged ? s.stop_test * T : opt->nIters;
+		res->seconds = t1 - t0;
+		res->pad_ = 0;
+	}
+	return 0;
+}
+
+/* ---------------------------------------------------------------- checks */
+
+int nmfk_job_dot_kl(nmfk_job *j, double *dot_V, double *dot_VWH, double *kl)
+{
+	double *h = (double *)malloc(sizeof(double) * 3 * j->ndot);
+	if (!h) {
+		nmfk_set_error("out of host memory");
+		return NMFK_ENOMEM;
+	}
+	int e = nmfk_launch_dot_kl(j->V, j->ldv, j->W, j->H, j->n, j->m, j->k, j->kt, j->dot, j->stream);
+	if (!e)
+		e = hipMemcpyAsync(h, j->dot, sizeof(double) * 3 * j->ndot, hipMemcpyDeviceToHost, j->stream);
+	if (!e)
+		e = hipStreamSynchronize(j->stream);
+	if (e) {
+		free(h);
+		return nmfk_hip_fail(e, "dot/KL");
+	}
+	double s[3] = {0.0, 0.0, 0.0};
+	for (uint32_t b = 0; b < j->ndot; b++)
+		for (int q = 0; q < 3; q++)
+			s[q] += h[3 * (size_t)b + q];
+	free(h);
+	if (dot_V)
+		*dot_V = s[0];
+	if (dot_VWH)
+		*dot_VWH = s[1];
+	if (kl)
+		*kl = s[2];
+	return 0;
+}
+
+/* ---------------------------------------------------------------- timing */
+
+static int time_loop(nmfk_job *j, uint32_t iters, int kind, double *ms)
+{
+	hipEvent_t a, b;
+	int e = hipEventCreate(&a);
+	if (!e)
+		e = hipEventCreate(&b);
+	if (e)
+		return nmfk_hip_fail(e, "hipEventCreate");
+	e = hipEventRecord(a, j->stream);
+	for (uint32_t i = 0; i < iters && !e; i++) {
+		switch (kind) {
+		case 0:
+			e = nmfk_enqueue_half_H(j, NULL);
+			if (!e)
+				e = nmfk_enqueue_half_W(j, NULL, 0);
+			break;
+		case 1:
+			e = nmfk_launch_partial(j->V, j->ldv, j->H, j->W, j->m, j->n, j->kt, &j->planH, j->part, NULL,
+						j->stream);
+			break;
+		case 2:
+			e = nmfk_launch_finish(j->part, j->planH.S, j->H, j->m, j->k, j->kt, j->colW, j->ncolW,
+					       j->colH, 0, NULL, j->stream);
+			break;
+		case 3:
+			e = nmfk_launch_partial(j->Vt, j->ldvt, j->W, j->H, j->n, j->m, j->kt, &j->planW, j->part,
+						NULL, j->stream);
+			break;
+		case 4:
+			e = nmfk_launch_finish(j->part, j->planW.S, j->W, j->n, j->k, j->kt, j->colH, j->ncolH,
+					       j->colW, 0, NULL, j->stream);
+			break;
+		}
+	}
+	if (!e)
+		e = hipEventRecord(b, j->stream);
+	if (!e)
+		e = hipEventSynchronize(b);
+	float f = 0.0f;
+	if (!e)
+		e = hipEventElapsedTime(&f, a, b);
+	hipEventDestroy(a);
+	hipEventDestroy(b);
+	*ms = (double)f / (iters ? iters : 1);
+	return nmfk_hip_fail(e, "timing");
+}
+
+int nmfk_job_time_iterations(nmfk_job *j, uint32_t iters, nmfk_timing *t)
+{
+	int e = nmfk_launch_colsum_parts(j->W, j->n, j->kt, j->colW, j->stream);
+	if (e)
+		return nmfk_hip_fail(e, "timing setup");
+	double ms[5];
+	for (int kind = 0; kind < 5; kind++) {
+		e = time_loop(j, iters, kind, &ms[kind]);
+		if (e)
+			return e;
+	}
+	t->ms_per_iter = ms[0];
+	t->ms_partial_H = ms[1];
+	t->ms_finish_H = ms[2];
+	t->ms_partial_W = ms[3];
+	t->ms_finish_W = ms[4];
+	j->colW_valid = 0;
+	return 0;
+}

@@ -1,0 +1,46 @@
+/*
+ * nmfk_internal.h -- private definitions shared by the C host files of libnmfgpu_amd.
+ */
+#ifndef NMFK_INTERNAL_H
+#define NMFK_INTERNAL_H
+
+#include <hip/hip_runtime_api.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "nmfk.h"
+#include "nmfk_kernels.h"
+
+#define NMFK_EINVAL 2
+#define NMFK_EHIP 3
+#define NMFK_ENOMEM 4
+#define NMFK_EBUDGET 5
+
+struct nmfk_job {
+	uint32_t n, m, k, kt;
+	size_t ldv, ldvt;
+	float *V, *Vt, *W, *H;         /* device */
+	float *part;                   /* partial contractions, S x p x kt */
+	float *colW, *colH;            /* column-sum partials, one row per 256 rows */
+	uint32_t ncolW, ncolH;
+	double *dot;                   /* FP64 partial sums of the Frobenius / KL kernel */
+	uint32_t ndot;
+	uint32_t *cls, *last;          /* device class vectors */
+	nmfk_state *st;                /* device stop-rule state */
+	nmfk_state *h_st;              /* pinned, 2 slots */
+	nmfk_hplan planH, planW;
+	hipStream_t stream;
+	hipEvent_t ev[2];
+	hipGraphExec_t chunk_exec;
+	uint32_t chunk_T;
+	int colW_valid;
+};
+
+void nmfk_set_error(const char *fmt, ...);
+int nmfk_hip_fail(int err, const char *what);
+double nmfk_now(void);
+
+int nmfk_enqueue_half_H(nmfk_job *j, const nmfk_state *st);
+int nmfk_enqueue_half_W(nmfk_job *j, const nmfk_state *st, int clamp);
+
+#endif /* NMFK_INTERNAL_H */

@@ -1,0 +1,124 @@
+/*
+ * nmfk_kernels.h -- C prototypes of the gfx950 kernel launchers (nmfk_kernels.hip).
+ *
+ * Every launcher is asynchronous on <stream>, takes plain device pointers and sizes,
+ * and returns 0 or a hipError_t value.  Matrices are row-major FP32:
+ *   V  n x ldv   (ldv >= m, ldv % 4 == 0)        Vt  m x ldvt (ldvt >= n, ldvt % 4 == 0)
+ *   W  n x kt    H  m x kt   (H stored transposed, as the reference stores it)
+ * kt is the padded rank (nmfk_kt(k)); columns k..kt-1 of W and H are kept at zero.
+ */
+#ifndef NMFK_KERNELS_H
+#define NMFK_KERNELS_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Device-side convergence state of one factorization (src/NMF_GPU.c:522-711). */
+typedef struct nmfk_state {
+	uint32_t inc;         /* unchanged tests in a row */
+	uint32_t converged;   /* set by the test that meets the stop rule; every later launch is a no-op */
+	uint32_t ntests;      /* tests performed */
+	uint32_t stop_test;   /* 1-based test index that converged (0 = none) */
+	uint64_t last_diff;   /* connectivity difference of the most recent test */
+	uint32_t stop_threshold;
+	uint32_t pad_;
+} nmfk_state;
+
+/* Largest rank handled by the kernels. */
+#define NMFK_KMAX 64
+
+/* Padded rank used for storage and kernel selection. */
+uint32_t nmfk_kt(uint32_t k);
+
+/* Rows of A whose column sums one partial row covers (k_finish / k_colsum_parts). */
+#define NMFK_FIN_ROWS 256
+
+/* Launch plan of one half-step: grid of the partial kernel. */
+typedef struct nmfk_hplan {
+	uint32_t nab;     /* a-blocks (grid.x) */
+	uint32_t S;       /* b-splits (grid.y) */
+	uint32_t bchunk;  /* rows of U per b-split */
+	uint32_t ap;      /* a's per lane */
+	uint32_t nw;      /* waves per workgroup */
+} nmfk_hplan;
+
+/* Chooses the partial-kernel grid for updating a p x kt factor against q rows of U. */
+void nmfk_plan_half(uint32_t p, uint32_t q, uint32_t kt, nmfk_hplan *pl);
+
+/*
+ * Half-step, first kernel: part[s][a][c] = sum over b in split s of
+ *   B[b][c] * U[b][a] / (sum_c' A[a][c'] * B[b][c'])
+ * H half: U = V (q = n, p = m), A = H, B = W.   W half: U = Vt (q = m, p = n), A = W, B = H.
+ */
+int nmfk_launch_partial(const float *U, size_t ldu, const float *A, const float *B,
+			uint32_t p, uint32_t q, uint32_t kt, const nmfk_hplan *pl,
+			float *part, const nmfk_state *st, hipStream_t stream);
+
+/*
+ * Half-step, second kernel:
+ *   A[a][c] = (A[a][c] * sum_s part[s][a][c]) / sum_g colB[g][c]     for c < k
+ * then optionally the R_MIN clamp, then colA[blk][c] = column sums of the new A over
+ * each block of NMFK_FIN_ROWS rows (the next half-step's denominators).
+ */
+int nmfk_launch_finish(const float *part, uint32_t S, float *A, uint32_t p, uint32_t k, uint32_t kt,
+		       const float *colB, uint32_t ncolB, float *colA, int clamp,
+		       const nmfk_state *st, hipStream_t stream);
+
+/* colA[blk][c] = sum of A[a][c] over the rows of block blk (same order as k_finish). */
+int nmfk_launch_colsum_parts(const float *A, uint32_t p, uint32_t kt, float *colA, hipStream_t stream);
+
+/* out[c] = sum_g parts[g][c], g ascending. */
+int nmfk_launch_sum_parts(const float *parts, uint32_t nparts, uint32_t kt, uint32_t width,
+			  float *out, hipStream_t stream);
+
+/*
+ * Convergence test (src/NMF_GPU.c:611-708): clamps H (m x k, pitch kt) in place, writes
+ * cls[j] = argmax with the reference tie rule, counts the connectivity difference against
+ * last[] exactly (contingency table), copies cls into last and applies the stop rule.
+ */
+int nmfk_launch_test(float *H, uint32_t m, uint32_t k, uint32_t kt, uint32_t *cls, uint32_t *last,
+		     nmfk_state *st, hipStream_t stream);
+
+/* Resets a state block and zeroes last[] (the reference's initial all-zeros vector). */
+int nmfk_launch_state_reset(nmfk_state *st, uint32_t stop_threshold, uint32_t *last, uint32_t m,
+			    hipStream_t stream);
+
+/* A[i][j] = uniform (0,1] from Philox4x32-10 (seed, tag, i*width + j); padding columns = 0. */
+int nmfk_launch_uniform(float *A, uint32_t height, uint32_t width, uint32_t pitch,
+			uint64_t seed, uint32_t tag, hipStream_t stream);
+
+/* Synthetic V generators (benchmark inputs). */
+int nmfk_launch_gen_lognormal(float *V, uint32_t n, uint32_t m, uint32_t ldv, uint64_t seed,
+			      double mu, double sigma, hipStream_t stream);
+int nmfk_launch_gen_planted(float *V, uint32_t n, uint32_t m, uint32_t ldv, uint32_t kstar,
+			    uint64_t seed, double sigma, double *scratch, hipStream_t stream);
+
+/* Vt[j][i] = V[i][j]. */
+int nmfk_launch_transpose(const float *V, uint32_t n, uint32_t m, size_t ldv, float *Vt, size_t ldvt,
+			  hipStream_t stream);
+
+/* Per-block FP64 partial sums {sum V^2, sum (V-WH)^2, sum V log(V/WH) - V + WH}. */
+uint32_t nmfk_dot_kl_blocks(uint32_t n, uint32_t m);
+int nmfk_launch_dot_kl(const float *V, size_t ldv, const float *W, const float *H, uint32_t n,
+		       uint32_t m, uint32_t k, uint32_t kt, double *parts, hipStream_t stream);
+
+/* Elementwise / row ops of the reference's matrix_operations.h. */
+int nmfk_launch_div_sub(float *A, const float *B, size_t nitems, int div_operator, hipStream_t stream);
+int nmfk_launch_mul_div(float *A, const float *Aux, const float *acc, uint32_t height, uint32_t width,
+			uint32_t pitch, hipStream_t stream);
+int nmfk_launch_adjust(float *A, uint32_t height, uint32_t width, uint32_t pitch, hipStream_t stream);
+int nmfk_launch_idx_max(const float *A, uint32_t height, uint32_t width, uint32_t pitch, uint32_t *idx,
+			hipStream_t stream);
+int nmfk_launch_colsum_any(const float *A, uint32_t height, uint32_t width, uint32_t pitch,
+			   float *acc, hipStream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NMFK_KERNELS_H */

@@ -1,0 +1,941 @@
+// nmfk_kernels.hip -- hand-written gfx950 kernels for the KL multiplicative-update loop
+// of NMF-mGPU (update_H / update_W, src/NMF_routines.c:704-1000 and :1344-1636).
+//
+// Data layout in HBM (all FP32, row-major):
+//   V  n x ldv, and its transpose Vt  m x ldvt, both resident for the whole run, so that
+//   each half-step streams its operand with fully coalesced rows;
+//   W  n x kt, H  m x kt (H transposed, as in the reference), kt = padded rank.
+//
+// A half-step "updates factor A (p x kt) against factor B (q x kt) through U (q x p)":
+//   H half: U = V,  A = H (p = m), B = W (q = n)     (src/NMF_routines.c:376-525)
+//   W half: U = Vt, A = W (p = n), B = H (q = m)     (src/NMF_routines.c:1014-1164)
+//   A[a][c] <- A[a][c] * (sum_b B[b][c] U[b][a] / (A[a] . B[b])) / (sum_b B[b][c])
+// The reference materialises WH and Aux (n x m) in HBM between cuBLAS GEMMs and two
+// elementwise kernels; here the product, the ratio and the contraction are one streaming
+// pass in registers: a lane owns AP consecutive rows a of the factor being updated, the
+// other factor's row B[b] is wave-uniform (scalar loads), and U[b][a..] is read once with
+// coalesced vector loads.  The b range is split over workgroups; partial sums are reduced
+// in a fixed order (no float atomics), so every result is bit-reproducible.
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "nmfk_kernels.h"
+
+#define NMFK_RMIN 1.17549435e-38f   // FLT_MIN, R_MIN of the FP32 build (include/real_type.h:120-126)
+
+// --------------------------------------------------------------------------------------
+// Philox4x32-10 (Salmon et al., SC'11).  Identical to the host generator of the oracle.
+// --------------------------------------------------------------------------------------
+
+#define PHILOX_M0 0xD2511F53u
+#define PHILOX_M1 0xCD9E8D57u
+#define PHILOX_W0 0x9E3779B9u
+#define PHILOX_W1 0xBB67AE85u
+#define NMFK_DOMAIN 0x4E4D4621u
+#define TAG_NOISE 0x100u
+#define TAG_WSTAR 0x101u
+#define TAG_HSTAR 0x102u
+
+__device__ __forceinline__ void philox_block(uint64_t seed, uint32_t tag, uint64_t q, uint32_t out[4])
+{
+	uint32_t c0 = (uint32_t)q, c1 = (uint32_t)(q >> 32), c2 = tag, c3 = NMFK_DOMAIN;
+	uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+#pragma unroll
+	for (int r = 0; r < 10; r++) {
+		const uint64_t p0 = (uint64_t)PHILOX_M0 * c0;
+		const uint64_t p1 = (uint64_t)PHILOX_M1 * c2;
+		const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+		const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+		c0 = hi1 ^ c1 ^ k0;
+		c1 = lo1;
+		c2 = hi0 ^ c3 ^ k1;
+		c3 = lo0;
+		k0 += PHILOX_W0;
+		k1 += PHILOX_W1;
+	}
+	out[0] = c0;
+	out[1] = c1;
+	out[2] = c2;
+	out[3] = c3;
+}
+
+__device__ __forceinline__ float uniform01(uint64_t seed, uint32_t tag, uint64_t idx)
+{
+	uint32_t o[4];
+	philox_block(seed, tag, idx >> 2, o);
+	const uint32_t x = o[idx & 3u];
+	return (float)((x >> 8) + 1u) * (1.0f / 16777216.0f);
+}
+
+__device__ __forceinline__ double u53(uint32_t hi, uint32_t lo)
+{
+	const uint64_t b = ((uint64_t)(hi >> 5) << 26) | (uint64_t)(lo >> 6);
+	return ((double)b + 1.0) * (1.0 / 9007199254740992.0);
+}
+
+__device__ __forceinline__ double normal01(uint64_t seed, uint32_t tag, uint64_t idx)
+{
+	uint32_t o[4];
+	philox_block(seed, tag, idx, o);
+	const double u1 = u53(o[0], o[1]);
+	const double u2 = u53(o[2], o[3]);
+	return sqrt(-2.0 * log(u1)) * cos(6.283185307179586476925286766559 * u2);
+}
+
+__device__ __forceinline__ float clamp_rmin(float x)
+{
+	// (isnan(x) || x < R_MIN) ? R_MIN : x   (src/GPU_kernels.cu:2338-2345)
+	return (x >= NMFK_RMIN) ? x : NMFK_RMIN;
+}
+
+// Sum over a wave64 with a fixed xor tree; every lane ends with the same bits.
+__device__ __forceinline__ float wave_sum(float v)
+{
+#pragma unroll
+	for (int off = 32; off >= 1; off >>= 1)
+		v += __shfl_xor(v, off, 64);
+	return v;
+}
+
+__device__ __forceinline__ double wave_sum_d(double v)
+{
+#pragma unroll
+	for (int off = 32; off >= 1; off >>= 1)
+		v += __shfl_xor(v, off, 64);
+	return v;
+}
+
+// Sum of column c over nparts partial rows, g ascending (used everywhere such sums appear).
+__device__ __forceinline__ float sum_parts_col(const float *__restrict__ parts, int nparts, int ld, int c)
+{
+	float s = 0.0f;
+	for (int g = 0; g < nparts; g++)
+		s += parts[(size_t)g * ld + c];
+	return s;
+}
+
+// Column sums of a block of 256 rows (one row per thread, x[] = that row), written as one
+// partial row.  Shared by k_finish and k_colsum_parts so both give identical bits.
+template <int KT>
+__device__ __forceinline__ void block_colsum_store(float (&x)[KT], float *__restrict__ out)
+{
+	__shared__ float wsum[4][KT];
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+	for (int c = 0; c < KT; c++) {
+		const float s = wave_sum(x[c]);
+		if (lane == 0)
+			wsum[wave][c] = s;
+	}
+	__syncthreads();
+	for (int c = threadIdx.x; c < KT; c += 256)
+		out[c] = ((wsum[0][c] + wsum[1][c]) + wsum[2][c]) + wsum[3][c];
+}
+
+// --------------------------------------------------------------------------------------
+// Half-step kernels.
+// --------------------------------------------------------------------------------------
+
+template <int AP>
+struct vecf;
+template <>
+struct vecf<1> {
+	typedef float type;
+};
+template <>
+struct vecf<2> {
+	typedef float2 type;
+};
+template <>
+struct vecf<4> {
+	typedef float4 type;
+};
+
+template <int AP>
+__device__ __forceinline__ void load_u(const float *__restrict__ src, bool full, int avail, float (&u)[AP])
+{
+	if (full) {
+		if constexpr (AP == 4) {
+			const float4 t = *reinterpret_cast<const float4 *>(src);
+			u[0] = t.x, u[1] = t.y, u[2] = t.z, u[3] = t.w;
+		} else if constexpr (AP == 2) {
+			const float2 t = *reinterpret_cast<const float2 *>(src);
+			u[0] = t.x, u[1] = t.y;
+		} else {
+			u[0] = src[0];
+		}
+	} else {
+#pragma unroll
+		for (int r = 0; r < AP; r++)
+			u[r] = (r < avail) ? src[r] : 0.0f;
+	}
+}
+
+// part[blockIdx.y][a][c] = sum_{b in split} B[b][c] * U[b][a] / (A[a] . B[b])
+template <int KT, int AP, int NW>
+__global__ __launch_bounds__(64 * NW) void k_partial(const float *__restrict__ U, size_t ldu,
+						     const float *__restrict__ A,
+						     const float *__restrict__ B, int p, int q,
+						     int bchunk, float *__restrict__ part,
+						     const nmfk_state *__restrict__ st)
+{
+	if (st != nullptr && st->converged)
+		return;
+	const int lane = threadIdx.x & 63;
+	const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+	const int a0 = (blockIdx.x * 64 + lane) * AP;
+	const int avail = p - a0;
+	const bool full = avail >= AP;
+
+	float av[AP][KT], acc[AP][KT];
+#pragma unroll
+	for (int r = 0; r < AP; r++) {
+		const bool ok = r < avail;
+#pragma unroll
+		for (int c = 0; c < KT; c++) {
+			av[r][c] = ok ? A[(size_t)(a0 + r) * KT + c] : 0.0f;
+			acc[r][c] = 0.0f;
+		}
+	}
+
+	const int b_begin = blockIdx.y * bchunk;
+	const int b_end = min(q, b_begin + bchunk);
+	const float *Ucol = U + a0;
+#pragma unroll 2
+	for (int b = b_begin + wave; b < b_end; b += NW) {
+		float u[AP];
+		load_u<AP>(Ucol + (size_t)b * ldu, full, avail, u);
+		const float *Bb = B + (size_t)b * KT;
+		float bv[KT];
+#pragma unroll
+		for (int c = 0; c < KT; c++)
+			bv[c] = Bb[c];
+#pragma unroll
+		for (int r = 0; r < AP; r++) {
+			float wh = 0.0f;
+#pragma unroll
+			for (int c = 0; c < KT; c++)
+				wh = fmaf(av[r][c], bv[c], wh);
+			const float ratio = u[r] / wh;
+#pragma unroll
+			for (int c = 0; c < KT; c++)
+				acc[r][c] = fmaf(ratio, bv[c], acc[r][c]);
+		}
+	}
+
+	// Fixed-order tree over the NW waves of the workgroup.
+	constexpr int E = AP * KT;
+	__shared__ float red[(NW / 2 > 0 ? NW / 2 : 1) * E * 64];
+#pragma unroll
+	for (int half = NW / 2; half >= 1; half >>= 1) {
+		if (wave >= half && wave < 2 * half) {
+			float *dst = red + (size_t)(wave - half) * E * 64;
+#pragma unroll
+			for (int r = 0; r < AP; r++)
+#pragma unroll
+				for (int c = 0; c < KT; c++)
+					dst[(r * KT + c) * 64 + lane] = acc[r][c];
+		}
+		__syncthreads();
+		if (wave < half) {
+			const float *src = red + (size_t)wave * E * 64;
+#pragma unroll
+			for (int r = 0; r < AP; r++)
+#pragma unroll
+				for (int c = 0; c < KT; c++)
+					acc[r][c] += src[(r * KT + c) * 64 + lane];
+		}
+		__syncthreads();
+	}
+	if (wave == 0) {
+		float *dst = part + ((size_t)blockIdx.y * p + a0) * KT;
+#pragma unroll
+		for (int r = 0; r < AP; r++)
+			if (r < avail) {
+#pragma unroll
+				for (int c = 0; c < KT; c += 4)
+					if constexpr (KT % 4 == 0)
+						*reinterpret_cast<float4 *>(dst + r * KT + c) =
+							make_float4(acc[r][c], acc[r][c + 1], acc[r][c + 2], acc[r][c + 3]);
+			}
+	}
+}
+
+// A[a][c] = (A[a][c] * sum_s part[s][a][c]) / sum_g colB[g][c], optional clamp, then the
+// column sums of the new A per block of 256 rows.  (kernel_mul_div: multiply first, then
+// divide, src/GPU_kernels.cu:2029-2033.)
+template <int KT>
+__global__ __launch_bounds__(256) void k_finish(const float *__restrict__ part, int S,
+						float *__restrict__ A, int p, int k,
+						const float *__restrict__ colB, int ncolB,
+						float *__restrict__ colA, int clamp,
+						const nmfk_state *__restrict__ st)
+{
+	if (st != nullptr && st->converged)
+		return;
+	__shared__ float den[KT];
+	for (int c = threadIdx.x; c < KT; c += 256)
+		den[c] = sum_parts_col(colB, ncolB, KT, c);
+	__syncthreads();
+
+	const int a = blockIdx.x * 256 + threadIdx.x;
+	float x[KT];
+	if (a < p) {
+		const size_t pst = (size_t)p * KT;
+		const float *pa = part + (size_t)a * KT;
+		float s[KT];
+#pragma unroll
+		for (int c = 0; c < KT; c++)
+			s[c] = pa[c];
+		for (int sp = 1; sp < S; sp++) {
+			const float *ps = pa + sp * pst;
+#pragma unroll
+			for (int c = 0; c < KT; c++)
+				s[c] += ps[c];
+		}
+		float *Ar = A + (size_t)a * KT;
+#pragma unroll
+		for (int c = 0; c < KT; c++) {
+			if (c < k) {
+				float v = Ar[c] * s[c];
+				v = v / den[c];
+				if (clamp)
+					v = clamp_rmin(v);
+				x[c] = v;
+			} else {
+				x[c] = 0.0f;
+			}
+		}
+#pragma unroll
+		for (int c = 0; c < KT; c += 4)
+			*reinterpret_cast<float4 *>(Ar + c) = make_float4(x[c], x[c + 1], x[c + 2], x[c + 3]);
+	} else {
+#pragma unroll
+		for (int c = 0; c < KT; c++)
+			x[c] = 0.0f;
+	}
+	block_colsum_store<KT>(x, colA + (size_t)blockIdx.x * KT);
+}
+
+template <int KT>
+__global__ __launch_bounds__(256) void k_colsum_parts(const float *__restrict__ A, int p,
+						      float *__restrict__ colA)
+{
+	const int a = blockIdx.x * 256 + threadIdx.x;
+	float x[KT];
+#pragma unroll
+	for (int c = 0; c < KT; c++)
+		x[c] = (a < p) ? A[(size_t)a * KT + c] : 0.0f;
+	block_colsum_store<KT>(x, colA + (size_t)blockIdx.x * KT);
+}
+
+__global__ void k_sum_parts(const float *__restrict__ parts, int nparts, int ld, int width,
+			    float *__restrict__ out)
+{
+	const int c = blockIdx.x * blockDim.x + threadIdx.x;
+	if (c < width)
+		out[c] = sum_parts_col(parts, nparts, ld, c);
+}
+
+// --------------------------------------------------------------------------------------
+// Argmax with the reference's thread layout (src/GPU_kernels.cu:2592-2830,
+// src/matrix_operations.c:190-203): bw "threads" (lanes) each scan columns t, t+bw, t+2bw
+// with a strict '>' starting from their first column (or from (-1, 0) without one); a
+// tree over halves then keeps the lower lane unless lower < upper.  bw = 32 while the
+// reference pitch (width rounded up to 32) is <= 96, else next_pow2(ceil(pitch/3)).
+// One wave per row; widths up to 192 (bw <= 64).
+// --------------------------------------------------------------------------------------
+
+__host__ __device__ __forceinline__ int ref_block_width(int width)
+{
+	const int kp = (width + 31) & ~31;
+	if (kp <= 96)
+		return 32;
+	const int need = (kp + 2) / 3;
+	int bw = 1;
+	while (bw < need)
+		bw <<= 1;
+	return bw;
+}
+
+// Lane t (t < bw) holds v[i] = row[t + i*bw] for i < 3 (bw >= ceil(pitch/3) >= width/3,
+// so three items per lane always cover the row, as in the reference's 3 items per thread).
+// Returns the winning column in every lane.
+__device__ __forceinline__ int wave_argmax_ref(const float (&v)[3], int width, int bw, int lane)
+{
+	float mv = -1.0f;
+	int mi = 0;
+	if (lane < bw && lane < width) {
+		mv = v[0];
+		mi = lane;
+		if (lane + bw < width && v[1] > mv) {
+			mv = v[1];
+			mi = lane + bw;
+		}
+		if (lane + 2 * bw < width && v[2] > mv) {
+			mv = v[2];
+			mi = lane + 2 * bw;
+		}
+	}
+	for (int half = bw >> 1; half >= 1; half >>= 1) {
+		const float ov = __shfl_down(mv, half, 64);
+		const int oi = __shfl_down(mi, half, 64);
+		if (lane < half && mv < ov) {
+			mv = ov;
+			mi = oi;
+		}
+	}
+	return __shfl(mi, 0, 64);
+}
+
+__device__ __forceinline__ void load_row_items(const float *__restrict__ row, int width, int bw, int lane,
+					       float (&v)[3])
+{
+#pragma unroll
+	for (int i = 0; i < 3; i++) {
+		const int c = lane + i * bw;
+		v[i] = (lane < bw && c < width) ? row[c] : 0.0f;
+	}
+}
+
+__global__ __launch_bounds__(256) void k_idx_max(const float *__restrict__ A, int height, int width,
+						 int pitch, uint32_t *__restrict__ idx)
+{
+	const int lane = threadIdx.x & 63;
+	const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+	if (row >= height)
+		return;
+	const int bw = ref_block_width(width);
+	float v[3];
+	load_row_items(A + (size_t)row * pitch, width, bw, lane, v);
+	const int w = wave_argmax_ref(v, width, bw, lane);
+	if (lane == 0)
+		idx[row] = (uint32_t)w;
+}
+
+// --------------------------------------------------------------------------------------
+// Convergence test: adjust(H) + classification + connectivity difference + stop rule
+// (src/NMF_GPU.c:611-708).  One workgroup of 1024 threads; one wave per row of H.
+// The difference count  #{i<j : [c_i==c_j] != [l_i==l_j]}  equals
+//   sum_a C(r_a,2) + sum_b C(s_b,2) - 2 sum_ab C(n_ab,2)
+// over the contingency table n_ab = #{j : c_j == a, l_j == b}; integer and exact.
+// --------------------------------------------------------------------------------------
+
+__global__ __launch_bounds__(1024) void k_test(float *__restrict__ H, int m, int k, int kt,
+					       uint32_t *__restrict__ cls, uint32_t *__restrict__ last,
+					       nmfk_state *__restrict__ st)
+{
+	if (st->converged)
+		return;
+	__shared__ uint32_t tab[NMFK_KMAX * NMFK_KMAX];
+	__shared__ unsigned long long red[16];
+	for (int e = threadIdx.x; e < NMFK_KMAX * NMFK_KMAX; e += 1024)
+		tab[e] = 0;
+	__syncthreads();
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	const int bw = ref_block_width(k);
+	for (int j = wave; j < m; j += 16) {
+		float *row = H + (size_t)j * kt;
+		float v[3];
+		load_row_items(row, k, bw, lane, v);
+#pragma unroll
+		for (int i = 0; i < 3; i++) {
+			const int cc = lane + i * bw;
+			if (lane < bw && cc < k) {
+				v[i] = clamp_rmin(v[i]);
+				row[cc] = v[i];
+			}
+		}
+		const int c = wave_argmax_ref(v, k, bw, lane);
+		if (lane == 0) {
+			const uint32_t l = last[j];
+			cls[j] = (uint32_t)c;
+			last[j] = (uint32_t)c;
+			atomicAdd(&tab[c * NMFK_KMAX + (int)l], 1u);
+		}
+	}
+	__syncthreads();
+	// S_both - S_new/2 - S_old/2 accumulated as  2*C(n_ab,2) terms and marginal terms.
+	unsigned long long both = 0, rows = 0, cols = 0;
+	for (int e = threadIdx.x; e < NMFK_KMAX * NMFK_KMAX; e += 1024) {
+		const unsigned long long v = tab[e];
+		both += v * (v - (v > 0)) / 2;
+	}
+	if (threadIdx.x < NMFK_KMAX) {
+		unsigned long long r = 0, s = 0;
+		for (int b = 0; b < NMFK_KMAX; b++) {
+			r += tab[threadIdx.x * NMFK_KMAX + b];
+			s += tab[b * NMFK_KMAX + threadIdx.x];
+		}
+		rows = r * (r - (r > 0)) / 2;
+		cols = s * (s - (s > 0)) / 2;
+	}
+	// exact integer reductions (order-free)
+	unsigned long long t = rows + cols;
+	for (int off = 32; off >= 1; off >>= 1) {
+		both += __shfl_xor(both, off, 64);
+		t += __shfl_xor(t, off, 64);
+	}
+	__shared__ unsigned long long red2[16];
+	if (lane == 0) {
+		red[wave] = both;
+		red2[wave] = t;
+	}
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		unsigned long long B = 0, Tm = 0;
+		for (int w = 0; w < 16; w++) {
+			B += red[w];
+			Tm += red2[w];
+		}
+		const unsigned long long diff = Tm - 2ull * B;
+		st->last_diff = diff;
+		st->ntests += 1;
+		if (diff)
+			st->inc = 0;
+		else if (st->inc < st->stop_threshold)
+			st->inc += 1;
+		else {
+			st->converged = 1;
+			st->stop_test = st->ntests;
+		}
+	}
+}
+
+__global__ void k_state_reset(nmfk_state *st, uint32_t stop_threshold, uint32_t *last, int m)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i == 0) {
+		st->inc = 0;
+		st->converged = 0;
+		st->ntests = 0;
+		st->stop_test = 0;
+		st->last_diff = 0;
+		st->stop_threshold = stop_threshold;
+		st->pad_ = 0;
+	}
+	for (int j = i; j < m; j += gridDim.x * blockDim.x)
+		last[j] = 0;
+}
+
+// --------------------------------------------------------------------------------------
+// Initialisation and synthetic inputs.
+// --------------------------------------------------------------------------------------
+
+__global__ void k_uniform(float *__restrict__ A, uint32_t height, uint32_t width, uint32_t pitch,
+			  uint64_t seed, uint32_t tag)
+{
+	const size_t total = (size_t)height * pitch;
+	for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+	     e += (size_t)gridDim.x * blockDim.x) {
+		const uint32_t i = (uint32_t)(e / pitch), j = (uint32_t)(e % pitch);
+		A[e] = (j < width) ? uniform01(seed, tag, (uint64_t)i * width + j) : 0.0f;
+	}
+}
+
+__global__ void k_gen_lognormal(float *__restrict__ V, uint32_t n, uint32_t m, uint32_t ldv,
+				uint64_t seed, double mu, double sigma)
+{
+#pragma clang fp contract(off)
+	const size_t total = (size_t)n * ldv;
+	for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+	     e += (size_t)gridDim.x * blockDim.x) {
+		const uint32_t i = (uint32_t)(e / ldv), j = (uint32_t)(e % ldv);
+		V[e] = (j < m) ? (float)exp(mu + sigma * normal01(seed, TAG_NOISE, (uint64_t)i * m + j)) : 0.0f;
+	}
+}
+
+__global__ void k_gen_exp1(double *__restrict__ out, size_t count, uint64_t seed, uint32_t tag)
+{
+	for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < count;
+	     e += (size_t)gridDim.x * blockDim.x) {
+		uint32_t o[4];
+		philox_block(seed, tag, e, o);
+		out[e] = -log(u53(o[0], o[1]));
+	}
+}
+
+__global__ void k_gen_planted(float *__restrict__ V, uint32_t n, uint32_t m, uint32_t ldv, uint32_t kstar,
+			      const double *__restrict__ Ws, const double *__restrict__ Hs,
+			      uint64_t seed, double sigma)
+{
+#pragma clang fp contract(off)
+	const size_t total = (size_t)n * ldv;
+	for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+	     e += (size_t)gridDim.x * blockDim.x) {
+		const uint32_t i = (uint32_t)(e / ldv), j = (uint32_t)(e % ldv);
+		if (j >= m) {
+			V[e] = 0.0f;
+			continue;
+		}
+		double s = 0.0;
+		for (uint32_t c = 0; c < kstar; c++)
+			s += Ws[(size_t)i * kstar + c] * Hs[(size_t)c * m + j];
+		const double z = normal01(seed, TAG_NOISE, (uint64_t)i * m + j);
+		V[e] = (float)(s * exp(sigma * z));
+	}
+}
+
+// 64x64 tiles through LDS.
+__global__ __launch_bounds__(256) void k_transpose(const float *__restrict__ V, int n, int m, size_t ldv,
+						   float *__restrict__ Vt, size_t ldvt)
+{
+	__shared__ float tile[64][65];
+	const int i0 = blockIdx.y * 64, j0 = blockIdx.x * 64;
+	const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+	for (int r = ty; r < 64; r += 4) {
+		const int i = i0 + r, j = j0 + tx;
+		tile[r][tx] = (i < n && j < m) ? V[(size_t)i * ldv + j] : 0.0f;
+	}
+	__syncthreads();
+	for (int r = ty; r < 64; r += 4) {
+		const int j = j0 + r, i = i0 + tx;
+		if (j < m && i < n)
+			Vt[(size_t)j * ldvt + i] = tile[tx][r];
+	}
+}
+
+// --------------------------------------------------------------------------------------
+// FP64 Frobenius and KL sums (dot_product_VWH, src/NMF_routines.c:1758-2794, plus the
+// generalized KL objective).  Lane = column j; the block's row range is strided over.
+// --------------------------------------------------------------------------------------
+
+#define DOT_ROWSPLIT 64
+
+template <int KT>
+__global__ __launch_bounds__(256) void k_dot_kl(const float *__restrict__ V, size_t ldv,
+						const float *__restrict__ W, const float *__restrict__ H,
+						int n, int m, double *__restrict__ parts)
+{
+	const int j = blockIdx.x * 256 + threadIdx.x;
+	const int rows_per = (n + gridDim.y - 1) / gridDim.y;
+	const int i0 = blockIdx.y * rows_per, i1 = min(n, i0 + rows_per);
+	double hj[KT];
+#pragma unroll
+	for (int c = 0; c < KT; c++)
+		hj[c] = (j < m) ? (double)H[(size_t)j * KT + c] : 0.0;
+	double sv = 0.0, sd = 0.0, sk = 0.0;
+	if (j < m) {
+		for (int i = i0; i < i1; i++) {
+			const float *Wi = W + (size_t)i * KT;
+			double wh = 0.0;
+#pragma unroll
+			for (int c = 0; c < KT; c++)
+				wh += (double)Wi[c] * hj[c];
+			const double v = (double)V[(size_t)i * ldv + j];
+			const double d = v - wh;
+			sv += v * v;
+			sd += d * d;
+			sk += (v > 0.0 ? v * log(v / wh) : 0.0) - v + wh;
+		}
+	}
+	__shared__ double red[3][4];
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	sv = wave_sum_d(sv);
+	sd = wave_sum_d(sd);
+	sk = wave_sum_d(sk);
+	if (lane == 0) {
+		red[0][wave] = sv;
+		red[1][wave] = sd;
+		red[2][wave] = sk;
+	}
+	__syncthreads();
+	if (threadIdx.x < 3) {
+		const int q = threadIdx.x;
+		parts[((size_t)blockIdx.y * gridDim.x + blockIdx.x) * 3 + q] =
+			((red[q][0] + red[q][1]) + red[q][2]) + red[q][3];
+	}
+}
+
+// --------------------------------------------------------------------------------------
+// Elementwise and row kernels of the reference's matrix_operations.h.
+// --------------------------------------------------------------------------------------
+
+__global__ void k_div_sub(float *__restrict__ A, const float *__restrict__ B, size_t nitems, int div_op)
+{
+	for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < nitems;
+	     e += (size_t)gridDim.x * blockDim.x) {
+		const float a = A[e], b = B[e];
+		A[e] = div_op ? (b / a) : (b - a);   // A = B / A  or  A = B - A
+	}
+}
+
+__global__ void k_mul_div(float *__restrict__ A, const float *__restrict__ Aux, const float *__restrict__ acc,
+			  uint32_t height, uint32_t width, uint32_t pitch)
+{
+	const size_t total = (size_t)height * pitch;
+	for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+	     e += (size_t)gridDim.x * blockDim.x) {
+		const uint32_t c = (uint32_t)(e % pitch);
+		if (c < width) {
+			const float t = A[e] * Aux[e];
+			A[e] = t / acc[c];
+		}
+	}
+}
+
+__global__ void k_adjust(float *__restrict__ A, uint32_t height, uint32_t width, uint32_t pitch)
+{
+	const size_t total = (size_t)height * pitch;
+	for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+	     e += (size_t)gridDim.x * blockDim.x) {
+		const uint32_t c = (uint32_t)(e % pitch);
+		if (c < width)
+			A[e] = clamp_rmin(A[e]);
+	}
+}
+
+// Column sums of an arbitrary height x width matrix: parts[split][c], rows of split
+// summed ascending.
+__global__ void k_colsum_generic(const float *__restrict__ A, uint32_t height, uint32_t width, uint32_t pitch,
+				 uint32_t rows_per, float *__restrict__ parts)
+{
+	const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+	if (c >= width)
+		return;
+	const uint32_t i0 = blockIdx.y * rows_per;
+	const uint32_t i1 = min(height, i0 + rows_per);
+	float s = 0.0f;
+	for (uint32_t i = i0; i < i1; i++)
+		s += A[(size_t)i * pitch + c];
+	parts[(size_t)blockIdx.y * width + c] = s;
+}
+
+// --------------------------------------------------------------------------------------
+// Host-side launchers (extern "C").
+// --------------------------------------------------------------------------------------
+
+static inline unsigned grid1d(size_t total, unsigned block)
+{
+	size_t g = (total + block - 1) / block;
+	if (g > 65535u * 16u)
+		g = 65535u * 16u;
+	return (unsigned)(g ? g : 1);
+}
+
+#define NMFK_KT_LIST(X) X(4) X(8) X(12) X(16) X(20) X(24) X(28) X(32) X(40) X(48) X(56) X(64)
+
+extern "C" uint32_t nmfk_kt(uint32_t k)
+{
+	if (k <= 32)
+		return (k + 3u) & ~3u;
+	return (k + 7u) & ~7u;
+}
+
+static inline uint32_t default_ap(uint32_t kt)
+{
+	return kt <= 8 ? 4u : (kt <= 32 ? 2u : 1u);
+}
+
+extern "C" void nmfk_plan_half(uint32_t p, uint32_t q, uint32_t kt, nmfk_hplan *pl)
+{
+	const uint32_t ap = default_ap(kt);
+	const uint32_t nw = 8;
+	const uint32_t nab = (p + 64u * ap - 1u) / (64u * ap);
+	const uint32_t target = 512;  // workgroups of 8 waves: 2 per CU on 256 CUs
+	uint32_t S = (target + nab - 1u) / nab;
+	const uint32_t smax = (q + nw * 8u - 1u) / (nw * 8u);  // at least 8 rows of U per wave
+	if (S > smax)
+		S = smax;
+	if (S < 1)
+		S = 1;
+	uint32_t bchunk = (q + S - 1u) / S;
+	S = (q + bchunk - 1u) / bchunk;
+	pl->nab = nab;
+	pl->S = S;
+	pl->bchunk = bchunk;
+	pl->ap = ap;
+	pl->nw = nw;
+}
+
+template <int KT, int AP>
+static int launch_partial_t(const float *U, size_t ldu, const float *A, const float *B, uint32_t p,
+			    uint32_t q, const nmfk_hplan *pl, float *part, const nmfk_state *st,
+			    hipStream_t stream)
+{
+	dim3 grid(pl->nab, pl->S);
+	if (pl->nw != 8)
+		return (int)hipErrorInvalidValue;
+	hipLaunchKernelGGL((k_partial<KT, AP, 8>), grid, dim3(512), 0, stream, U, ldu, A, B, (int)p,
+			   (int)q, (int)pl->bchunk, part, st);
+	return (int)hipGetLastError();
+}
+
+extern "C" int nmfk_launch_partial(const float *U, size_t ldu, const float *A, const float *B, uint32_t p,
+				   uint32_t q, uint32_t kt, const nmfk_hplan *pl, float *part,
+				   const nmfk_state *st, hipStream_t stream)
+{
+	if (ldu % 4 != 0 || pl->ap != default_ap(kt))
+		return (int)hipErrorInvalidValue;
+	switch (kt) {
+#define X(K)                                                                                          \
+	case K:                                                                                       \
+		return launch_partial_t<K, (K <= 8 ? 4 : (K <= 32 ? 2 : 1))>(U, ldu, A, B, p, q, pl, part, \
+									      st, stream);
+		NMFK_KT_LIST(X)
+#undef X
+	default:
+		return (int)hipErrorInvalidValue;
+	}
+}
+
+extern "C" int nmfk_launch_finish(const float *part, uint32_t S, float *A, uint32_t p, uint32_t k, uint32_t kt,
+				  const float *colB, uint32_t ncolB, float *colA, int clamp,
+				  const nmfk_state *st, hipStream_t stream)
+{
+	const unsigned g = (p + NMFK_FIN_ROWS - 1) / NMFK_FIN_ROWS;
+	switch (kt) {
+#define X(K)                                                                                           \
+	case K:                                                                                        \
+		hipLaunchKernelGGL((k_finish<K>), dim3(g), dim3(256), 0, stream, part, (int)S, A, (int)p, \
+				   (int)k, colB, (int)ncolB, colA, clamp, st);                          \
+		return (int)hipGetLastError();
+		NMFK_KT_LIST(X)
+#undef X
+	default:
+		return (int)hipErrorInvalidValue;
+	}
+}
+
+extern "C" int nmfk_launch_colsum_parts(const float *A, uint32_t p, uint32_t kt, float *colA, hipStream_t stream)
+{
+	const unsigned g = (p + NMFK_FIN_ROWS - 1) / NMFK_FIN_ROWS;
+	switch (kt) {
+#define X(K)                                                                                      \
+	case K:                                                                                   \
+		hipLaunchKernelGGL((k_colsum_parts<K>), dim3(g), dim3(256), 0, stream, A, (int)p, colA); \
+		return (int)hipGetLastError();
+		NMFK_KT_LIST(X)
+#undef X
+	default:
+		return (int)hipErrorInvalidValue;
+	}
+}
+
+extern "C" int nmfk_launch_sum_parts(const float *parts, uint32_t nparts, uint32_t kt, uint32_t width, float *out,
+				     hipStream_t stream)
+{
+	hipLaunchKernelGGL(k_sum_parts, dim3((width + 63) / 64), dim3(64), 0, stream, parts, (int)nparts,
+			   (int)kt, (int)width, out);
+	return (int)hipGetLastError();
+}
+
+extern "C" int nmfk_launch_test(float *H, uint32_t m, uint32_t k, uint32_t kt, uint32_t *cls, uint32_t *last,
+				nmfk_state *st, hipStream_t stream)
+{
+	if (k > NMFK_KMAX || k == 0)
+		return (int)hipErrorInvalidValue;
+	hipLaunchKernelGGL(k_test, dim3(1), dim3(1024), 0, stream, H, (int)m, (int)k, (int)kt, cls, last, st);
+	return (int)hipGetLastError();
+}
+
+extern "C" int nmfk_launch_state_reset(nmfk_state *st, uint32_t stop_threshold, uint32_t *last, uint32_t m,
+				       hipStream_t stream)
+{
+	hipLaunchKernelGGL(k_state_reset, dim3(grid1d(m, 256)), dim3(256), 0, stream, st, stop_threshold, last,
+			   (int)m);
+	return (int)hipGetLastError();
+}
+
+extern "C" int nmfk_launch_uniform(float *A, uint32_t height, uint32_t width, uint32_t pitch, uint64_t seed,
+				   uint32_t tag, hipStream_t stream)
+{
+	hipLaunchKernelGGL(k_uniform, dim3(grid1d((size_t)height * pitch, 256)), dim3(256), 0, stream, A, height,
+			   width, pitch, seed, tag);
+	return (int)hipGetLastError();
+}
+
+extern "C" int nmfk_launch_gen_lognormal(float *V, uint32_t n, uint32_t m, uint32_t ldv, uint64_t seed, double mu,
+					 double sigma, hipStream_t stream)
+{
+	hipLaunchKernelGGL(k_gen_lognormal, dim3(grid1d((size_t)n * ldv, 256)), dim3(256), 0, stream, V, n, m, ldv,
+			   seed, mu, sigma);
+	return (int)hipGetLastError();
+}
+
+extern "C" int nmfk_launch_gen_planted(float *V, uint32_t n, uint32_t m, uint32_t ldv, uint32_t kstar,
+				       uint64_t seed, double sigma, double *scratch, hipStream_t stream)
+{
+	double *Ws = scratch, *Hs = scratch + (size_t)n * kstar;
+	hipLaunchKernelGGL(k_gen_exp1, dim3(grid1d((size_t)n * kstar, 256)), dim3(256), 0, stream, Ws,
+			   (size_t)n * kstar, seed, TAG_WSTAR);
+	hipLaunchKernelGGL(k_gen_exp1, dim3(grid1d((size_t)kstar * m, 256)), dim3(256), 0, stream, Hs,
+			   (size_t)kstar * m, seed, TAG_HSTAR);
+	hipLaunchKernelGGL(k_gen_planted, dim3(grid1d((size_t)n * ldv, 256)), dim3(256), 0, stream, V, n, m, ldv,
+			   kstar, Ws, Hs, seed, sigma);
+	return (int)hipGetLastError();
+}
+
+extern "C" int nmfk_launch_transpose(const float *V, uint32_t n, uint32_t m, size_t ldv, float *Vt, size_t ldvt,
+				     hipStream_t stream)
+{
+	dim3 grid((m + 63) / 64, (n + 63) / 64);
+	hipLaunchKernelGGL(k_transpose, grid, dim3(256), 0, stream, V, (int)n, (int)m, ldv, Vt, ldvt);
+	return (int)hipGetLastError();
+}
+
+extern "C" uint32_t nmfk_dot_kl_blocks(uint32_t n, uint32_t m)
+{
+	(void)n;
+	return ((m + 255) / 256) * DOT_ROWSPLIT;
+}
+
+extern "C" int nmfk_launch_dot_kl(const float *V, size_t ldv, const float *W, const float *H, uint32_t n,
+				  uint32_t m, uint32_t k, uint32_t kt, double *parts, hipStream_t stream)
+{
+	(void)k;
+	dim3 grid((m + 255) / 256, DOT_ROWSPLIT);
+	switch (kt) {
+#define X(K)                                                                                      \
+	case K:                                                                                   \
+		hipLaunchKernelGGL((k_dot_kl<K>), grid, dim3(256), 0, stream, V, ldv, W, H, (int)n, (int)m, \
+				   parts);                                                        \
+		return (int)hipGetLastError();
+		NMFK_KT_LIST(X)
+#undef X
+	default:
+		return (int)hipErrorInvalidValue;
+	}
+}
+
+extern "C" int nmfk_launch_div_sub(float *A, const float *B, size_t nitems, int div_operator, hipStream_t stream)
+{
+	hipLaunchKernelGGL(k_div_sub, dim3(grid1d(nitems, 256)), dim3(256), 0, stream, A, B, nitems, div_operator);
+	return (int)hipGetLastError();
+}
+
+extern "C" int nmfk_launch_mul_div(float *A, const float *Aux, const float *acc, uint32_t height, uint32_t width,
+				   uint32_t pitch, hipStream_t stream)
+{
+	hipLaunchKernelGGL(k_mul_div, dim3(grid1d((size_t)height * pitch, 256)), dim3(256), 0, stream, A, Aux, acc,
+			   height, width, pitch);
+	return (int)hipGetLastError();
+}
+
+extern "C" int nmfk_launch_adjust(float *A, uint32_t height, uint32_t width, uint32_t pitch, hipStream_t stream)
+{
+	hipLaunchKernelGGL(k_adjust, dim3(grid1d((size_t)height * pitch, 256)), dim3(256), 0, stream, A, height,
+			   width, pitch);
+	return (int)hipGetLastError();
+}
+
+extern "C" int nmfk_launch_idx_max(const float *A, uint32_t height, uint32_t width, uint32_t pitch, uint32_t *idx,
+				   hipStream_t stream)
+{
+	if (width == 0 || width > 192 || width > pitch)
+		return (int)hipErrorInvalidValue;
+	hipLaunchKernelGGL(k_idx_max, dim3((height + 3) / 4), dim3(256), 0, stream, A, (int)height, (int)width,
+			   (int)pitch, idx);
+	return (int)hipGetLastError();
+}
+
+extern "C" int nmfk_launch_colsum_any(const float *A, uint32_t height, uint32_t width, uint32_t pitch, float *acc,
+				      hipStream_t stream)
+{
+	// Two stages through a temporary part of <acc>'s caller-provided scratch is avoided:
+	// one column per thread, all rows ascending (deterministic, no scratch).
+	hipLaunchKernelGGL(k_colsum_generic, dim3((width + 63) / 64, 1), dim3(64), 0, stream, A, height, width,
+			   pitch, height ? height : 1, acc);
+	return (int)hipGetLastError();
+}

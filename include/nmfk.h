@@ -1,0 +1,118 @@
+/*
+ * nmfk.h -- job-handle C-ABI of the MI355X KL-NMF library (libnmfgpu_amd.so).
+ *
+ * The reference keeps one factorization in process globals (include/common.h:222-245,
+ * include/GPU_setup.h:236-248) and drives it from nmf() (src/NMF_GPU.c:312-924).  This
+ * header exposes the same loop on an explicit handle, so that several factorizations can
+ * live on one GPU at once (rank sweeps, the serving loop).  The drop-in headers
+ * NMF_routines.h, matrix_operations.h and GPU_setup.h are implemented on top of it.
+ *
+ * Conventions: plain pointers and sizes; host matrices are row-major FP32 with an explicit
+ * leading dimension; H is m x k (transposed, as in the reference).  Functions return 0 on
+ * success and a nonzero code on failure, with a message retrievable by nmfk_last_error().
+ * A handle is used from one host thread at a time.
+ */
+#ifndef NMFK_H
+#define NMFK_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct nmfk_job nmfk_job;
+
+/* Options of the loop, as the reference's -k/-i/-j/-t flags (src/common.c:847-1119,
+ * defaults include/common_defaults.h:78-96). */
+typedef struct nmfk_options {
+	uint32_t k;              /* factorization rank, 1..64 */
+	uint32_t nIters;         /* maximum iterations (default 2000) */
+	uint32_t niter_test_conv;/* iterations between convergence tests (default 10) */
+	uint32_t stop_threshold; /* unchanged tests needed, minus one (default 40) */
+} nmfk_options;
+
+/* Outcome of nmfk_job_run. */
+typedef struct nmfk_result {
+	uint32_t iterations;     /* as the reference prints: tests*T when converged, else nIters */
+	uint32_t converged;      /* 1 if the stop rule was met */
+	uint32_t ntests;         /* convergence tests performed */
+	uint32_t pad_;
+	uint64_t last_diff;      /* connectivity difference of the last test */
+	double seconds;          /* wall time of the iteration loop (device work included) */
+} nmfk_result;
+
+/* Fills <opt> with the reference defaults (k=2, 2000, 10, 40). */
+void nmfk_default_options(nmfk_options *opt);
+
+/* Last error message of the calling thread ("" if none). */
+const char *nmfk_last_error(void);
+
+/* Selects the device used by subsequent calls of this thread (hipSetDevice). */
+int nmfk_set_device(int dev);
+
+/* Creates a job for an n x m input and rank k; allocates every device buffer it needs. */
+nmfk_job *nmfk_job_create(uint32_t n, uint32_t m, uint32_t k);
+void nmfk_job_destroy(nmfk_job *job);
+
+/* Device bytes a job of this shape allocates (used for admission by HBM budget). */
+size_t nmfk_job_bytes(uint32_t n, uint32_t m, uint32_t k);
+
+/* Input matrix V: from host memory (row-major, leading dimension ldv >= m), from device
+ * memory, or generated on the device (synthetic benchmark inputs).  All values must be
+ * non-negative; the transposed copy the W half-step streams is built here. */
+int nmfk_job_set_V(nmfk_job *job, const float *V, size_t ldv);
+int nmfk_job_set_V_device(nmfk_job *job, const float *dV, size_t ldv);
+int nmfk_job_gen_V_lognormal(nmfk_job *job, uint64_t seed, double mu, double sigma);
+int nmfk_job_gen_V_planted(nmfk_job *job, uint64_t seed, uint32_t kstar, double sigma);
+/* Copies V back to the host (ldv >= m). */
+int nmfk_job_get_V(nmfk_job *job, float *V, size_t ldv);
+
+/* Initial W (stream tag 0) and H (tag 1): uniform (0,1] from Philox4x32-10(seed). */
+int nmfk_job_init_random(nmfk_job *job, uint64_t seed);
+/* Initial W (n x k, ldw) and H (m x k, ldh) from host memory. */
+int nmfk_job_set_WH(nmfk_job *job, const float *W, size_t ldw, const float *H, size_t ldh);
+int nmfk_job_get_W(nmfk_job *job, float *W, size_t ldw);
+int nmfk_job_get_H(nmfk_job *job, float *H, size_t ldh);
+
+/* The whole loop of src/NMF_GPU.c:522-795 on the device. */
+int nmfk_job_run(nmfk_job *job, const nmfk_options *opt, nmfk_result *res);
+
+/* Single half-steps with the reference's structure (column sum, then the update). */
+int nmfk_job_update_H(nmfk_job *job);
+int nmfk_job_update_W(nmfk_job *job);
+/* R_MIN clamp of H and W (src/NMF_GPU.c:611-638). */
+int nmfk_job_adjust(nmfk_job *job);
+
+/* Class vector: argmax of every row of the current H with the reference tie rule. */
+int nmfk_job_get_classification(nmfk_job *job, uint32_t *cls);
+
+/* FP64 sums: <V,V>, <V-WH,V-WH> and the generalized KL divergence D(V||WH). */
+int nmfk_job_dot_kl(nmfk_job *job, double *dot_V, double *dot_VWH, double *kl);
+
+/* Synchronizes the job's stream. */
+int nmfk_job_sync(nmfk_job *job);
+
+/* Raw device pointers and shape (for the drop-in layer and for measurements). */
+typedef struct nmfk_job_info {
+	uint32_t n, m, k, kt;
+	size_t ldv, ldvt;
+	float *dV, *dVt, *dW, *dH;
+	void *stream;            /* hipStream_t */
+} nmfk_job_info;
+int nmfk_job_info_get(const nmfk_job *job, nmfk_job_info *info);
+
+/* Times <iters> full iterations (both half-steps, no tests) with HIP events on the job's
+ * stream; returns the average ms per iteration and per kernel kind. */
+typedef struct nmfk_timing {
+	double ms_per_iter;
+	double ms_partial_H, ms_finish_H, ms_partial_W, ms_finish_W;
+} nmfk_timing;
+int nmfk_job_time_iterations(nmfk_job *job, uint32_t iters, nmfk_timing *t);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NMFK_H */

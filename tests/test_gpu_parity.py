@@ -1,0 +1,192 @@
+"""GPU parity of the HIP path against the CPU oracle, through the C-ABI.
+
+Bars: class vectors, stop iterations and difference counts bit-exact; W, H and the KL
+divergence within 1e-3 relative of the FP64 oracle (single half-steps: 1e-5)."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, rel_err
+
+pytestmark = pytest.mark.gpu
+
+KS = [1, 2, 3, 4, 5, 7, 8, 10, 12, 16, 17, 20, 24, 28, 32, 33, 40, 48, 56, 64]
+
+
+def make_job(nmfk, orc, n, m, k, seed_v=1, seed_init=2, mu=0.0, sigma=1.0):
+    V = orc.gen_lognormal(n, m, seed_v, mu, sigma)
+    job = nmfk.Job(n, m, k)
+    job.set_V(V)
+    job.init_random(seed_init)
+    return job, V
+
+
+def test_uniform_init_is_bit_exact(nmfk, orc):
+    for n, m, k in [(100, 37, 3), (513, 64, 16), (70, 300, 33)]:
+        with nmfk.Job(n, m, k) as job:
+            job.init_random(77)
+            W0, H0 = orc.init_WH(n, m, k, 77)
+            assert np.array_equal(job.get_W(), W0)
+            assert np.array_equal(job.get_H(), H0)
+
+
+def test_V_roundtrip_and_generators(nmfk, orc):
+    n, m = 257, 131
+    V = orc.gen_lognormal(n, m, 3, 5.0, 1.5)
+    with nmfk.Job(n, m, 4) as job:
+        job.set_V(V)
+        assert np.array_equal(job.get_V(), V)
+        job.gen_V_lognormal(3, 5.0, 1.5)
+        assert rel_err(job.get_V(), V) < 1e-6          # device libm vs host libm, FP64 then rounded
+        job.gen_V_planted(4, 6, 0.3)
+        assert rel_err(job.get_V(), orc.gen_planted(n, m, 6, 4, 0.3)) < 1e-6
+
+
+@pytest.mark.parametrize("k", KS)
+def test_half_steps_match_oracle(nmfk, orc, k):
+    n, m = 700, 150
+    job, V = make_job(nmfk, orc, n, m, k, seed_v=k, seed_init=k + 100)
+    with job:
+        W0, H0 = orc.init_WH(n, m, k, k + 100)
+        job.update_H()
+        H1 = orc.update_H(V, W0, H0)
+        assert rel_err(job.get_H(), H1) < 1e-5
+        job.update_W()
+        W1 = orc.update_W(V, W0, H1)
+        assert rel_err(job.get_W(), W1) < 1e-5
+
+
+@pytest.mark.parametrize("n,m,k", [(1, 1, 1), (5, 3, 2), (63, 65, 3), (1000, 7, 7), (9, 2000, 2)])
+def test_half_steps_ragged_shapes(nmfk, orc, n, m, k):
+    job, V = make_job(nmfk, orc, n, m, k, seed_v=n + m, seed_init=k)
+    with job:
+        W0, H0 = orc.init_WH(n, m, k, k)
+        for _ in range(3):
+            job.update_H()
+            H0 = orc.update_H(V, W0, H0)
+            job.update_W()
+            W0 = orc.update_W(V, W0, H0)
+        assert rel_err(job.get_H(), H0) < 1e-4
+        assert rel_err(job.get_W(), W0) < 1e-4
+
+
+def test_golden_tiny_runs(nmfk, orc):
+    g = np.load(GOLDEN / "tiny.npz")
+    for ci in range(int(g["ncases"])):
+        p = f"c{ci}_"
+        n, m, k, sv, si = (int(x) for x in g[p + "shape"])
+        with nmfk.Job(n, m, k) as job:
+            job.set_V(g[p + "V"])
+            job.init_random(si)
+            r = job.run(200, 5, 3)
+            assert r["iterations"] == int(g[p + "iterations"]), ci
+            assert r["converged"] == bool(g[p + "converged"]) and r["ntests"] == int(g[p + "ntests"])
+            assert np.array_equal(job.get_classification(), g[p + "cls"]), ci
+
+
+def test_config0_parity(nmfk, orc):
+    # V 5000x38 lognormal(5, 1.5^2), k=3, T=10, stop_threshold=40, 2000 iterations.
+    g = np.load(GOLDEN / "config0.npz")
+    n, m, k, sv, si = (int(x) for x in g["params"])
+    V = orc.gen_lognormal(n, m, sv, 5.0, 1.5)
+    with nmfk.Job(n, m, k) as job:
+        job.set_V(V)
+        job.init_random(si)
+        r = job.run(2000, 10, 40)
+        assert r["converged"] and r["iterations"] == int(g["f64_iterations"])
+        assert np.array_equal(job.get_classification(), g["f64_cls"])
+        assert rel_err(job.get_H(), g["f64_H"]) < 1e-3
+        assert rel_err(job.get_W()[:64], g["f64_W_head"]) < 1e-3
+        _, _, kl = job.dot_kl()
+        assert abs(kl - float(g["f64_kl"])) / float(g["f64_kl"]) < 1e-3
+
+
+def test_run_equals_reference_structured_steps(nmfk, orc):
+    # nmfk_job_run (graph-captured intervals, column sums fused into the finish kernel,
+    # clamps in-register) must give the same bits as update_H/update_W/adjust.
+    n, m, k, T = 900, 120, 6, 4
+    V = orc.gen_lognormal(n, m, 5, 0.0, 1.0)
+    with nmfk.Job(n, m, k) as a, nmfk.Job(n, m, k) as b:
+        for j in (a, b):
+            j.set_V(V)
+            j.init_random(9)
+        a.run(3 * T + 2, T, 1000)
+        for _ in range(3):
+            for _ in range(T):
+                b.update_H()
+                b.update_W()
+            b.adjust()
+        for _ in range(2):
+            b.update_H()
+            b.update_W()
+        assert np.array_equal(a.get_W(), b.get_W())
+        assert np.array_equal(a.get_H(), b.get_H())
+
+
+def test_runs_are_deterministic(nmfk, orc):
+    n, m, k = 3000, 400, 16
+    V = orc.gen_lognormal(n, m, 6, 0.0, 1.0)
+    outs = []
+    for _ in range(2):
+        with nmfk.Job(n, m, k) as j:
+            j.set_V(V)
+            j.init_random(1)
+            j.run(40, 10, 1000)
+            outs.append((j.get_W(), j.get_H()))
+    assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
+
+
+def test_dot_and_kl_match_oracle(nmfk, orc):
+    n, m, k = 500, 90, 5
+    job, V = make_job(nmfk, orc, n, m, k)
+    with job:
+        job.run(30, 10, 100)
+        W, H = job.get_W(), job.get_H()
+        dv, dvwh, kl = job.dot_kl()
+        odv, odvwh = orc.dot_VWH(V.astype(np.float64), W.astype(np.float64), H.astype(np.float64))
+        okl = orc.kl_div(V, W, H, fp64=True)
+        assert abs(dv - odv) / odv < 1e-12
+        assert abs(dvwh - odvwh) / odvwh < 1e-9
+        assert abs(kl - okl) / okl < 1e-9
+
+
+def test_stop_rule_on_device_matches_oracle_trace(nmfk, orc):
+    # Per-test differences counted on the device (contingency table) reproduce the
+    # oracle's pairwise count through the whole run, including the reset cases.
+    n, m, k = 400, 60, 4
+    V = orc.gen_lognormal(n, m, 12, 0.0, 1.0)
+    W0, H0 = orc.init_WH(n, m, k, 13)
+    o = orc.nmf(V, W0, H0, 600, 3, 6, fp64=True)
+    with nmfk.Job(n, m, k) as job:
+        job.set_V(V)
+        job.init_random(13)
+        r = job.run(600, 3, 6)
+        assert r["iterations"] == o["iterations"] and r["ntests"] == o["ntests"]
+        assert np.array_equal(job.get_classification(), o["cls"])
+
+
+def test_full_size_config1_shape(nmfk, orc):
+    # 20000 x 1000, k = 16, planted rank 16: two iterations against the FP64 oracle, then
+    # size-independent properties over 30 more (KL non-increasing, factors finite >= 0).
+    n, m, k = 20000, 1000, 16
+    V = orc.gen_planted(n, m, 16, 21, 0.3)
+    W0, H0 = orc.init_WH(n, m, k, 22)
+    with nmfk.Job(n, m, k) as job:
+        job.set_V(V)
+        job.init_random(22)
+        W, H = W0.astype(np.float64), H0.astype(np.float64)
+        for _ in range(2):
+            job.update_H()
+            job.update_W()
+            H = orc.update_H(V, W, H)
+            W = orc.update_W(V, W, H)
+        assert rel_err(job.get_H(), H) < 1e-4 and rel_err(job.get_W(), W) < 1e-4
+        prev = job.dot_kl()[2]
+        for _ in range(3):
+            job.run(10, 10, 1000)
+            cur = job.dot_kl()[2]
+            assert cur <= prev * (1 + 1e-6)
+            prev = cur
+        for A in (job.get_W(), job.get_H()):
+            assert np.all(np.isfinite(A)) and A.min() >= 0

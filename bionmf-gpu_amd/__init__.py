@@ -109,6 +109,8 @@ def load(path: os.PathLike | str | None = None):
     _sig(lib, "nmfk_job_sync", i32, [vp])
     _sig(lib, "nmfk_job_info_get", i32, [vp, C.POINTER(JobInfo)])
     _sig(lib, "nmfk_job_time_iterations", i32, [vp, u32, C.POINTER(Timing)])
+    _sig(lib, "nmfk_job_set_kernel_path", i32, [vp, i32])
+    _sig(lib, "nmfk_job_kernel_path", i32, [vp])
     _lib = lib
     return lib
 
@@ -233,6 +235,14 @@ class Job:
 
     def sync(self):
         _check(self.lib.nmfk_job_sync(self.h), "sync")
+
+    PATH_AUTO, PATH_VALU, PATH_MFMA32 = 0, 1, 2
+
+    def set_kernel_path(self, path: int):
+        _check(self.lib.nmfk_job_set_kernel_path(self.h, path), "set_kernel_path")
+
+    def kernel_path(self) -> int:
+        return self.lib.nmfk_job_kernel_path(self.h)
 
     def info(self) -> JobInfo:
         inf = JobInfo()

@@ -71,7 +71,7 @@ typedef struct job_layout {
 	uint32_t kt;
 	size_t ldv, ldvt;
 	nmfk_hplan planH, planW;
-	size_t n_V, n_Vt, n_W, n_H, n_part, n_colW, n_colH, n_dot;
+	size_t n_V, n_Vt, n_W, n_H, n_part, n_colW, n_colH, n_den, n_dot;
 } job_layout;
 
 static void layout_of(uint32_t n, uint32_t m, uint32_t k, job_layout *L)
@@ -85,10 +85,21 @@ static void layout_of(uint32_t n, uint32_t m, uint32_t k, job_layout *L)
 	L->n_Vt = (size_t)m * L->ldvt;
 	L->n_W = (size_t)n * L->kt;
 	L->n_H = (size_t)m * L->kt;
-	size_t pH = (size_t)L->planH.S * m * L->kt, pW = (size_t)L->planW.S * n * L->kt;
-	L->n_part = pH > pW ? pH : pW;
+	/* partial buffer: large enough for every kernel path a job may be switched to */
+	L->n_part = 0;
+	for (int path = NMFK_PATH_VALU; path <= NMFK_PATH_MFMA32; path++) {
+		nmfk_hplan h, w;
+		nmfk_plan_half_path(m, n, L->kt, path, &h);
+		nmfk_plan_half_path(n, m, L->kt, path, &w);
+		size_t pH = (size_t)h.S * m * L->kt, pW = (size_t)w.S * n * L->kt;
+		if (pH > L->n_part)
+			L->n_part = pH;
+		if (pW > L->n_part)
+			L->n_part = pW;
+	}
 	L->n_colW = (size_t)((n + NMFK_FIN_ROWS - 1) / NMFK_FIN_ROWS) * L->kt;
 	L->n_colH = (size_t)((m + NMFK_FIN_ROWS - 1) / NMFK_FIN_ROWS) * L->kt;
+	L->n_den = 2 * (size_t)L->kt;
 	L->n_dot = (size_t)nmfk_dot_kl_blocks(n, m) * 3;
 }
 
@@ -96,7 +107,7 @@ size_t nmfk_job_bytes(uint32_t n, uint32_t m, uint32_t k)
 {
 	job_layout L;
 	layout_of(n, m, k, &L);
-	size_t b = 4 * (L.n_V + L.n_Vt + L.n_W + L.n_H + L.n_part + L.n_colW + L.n_colH);
+	size_t b = 4 * (L.n_V + L.n_Vt + L.n_W + L.n_H + L.n_part + L.n_colW + L.n_colH + L.n_den);
 	b += 8 * L.n_dot + 4 * 2 * (size_t)m + sizeof(nmfk_state) + 7 * 256;
 	return b;
 }
@@ -141,6 +152,8 @@ nmfk_job *nmfk_job_create(uint32_t n, uint32_t m, uint32_t k)
 	if (!e) e |= dalloc((void **)&j->part, 4 * L.n_part, "hipMalloc(partials)");
 	if (!e) e |= dalloc((void **)&j->colW, 4 * L.n_colW, "hipMalloc(colW)");
 	if (!e) e |= dalloc((void **)&j->colH, 4 * L.n_colH, "hipMalloc(colH)");
+	if (!e) e |= dalloc((void **)&j->denW, 4 * L.n_den, "hipMalloc(den)");
+	if (!e) j->denH = j->denW + L.kt;
 	if (!e) e |= dalloc((void **)&j->dot, 8 * L.n_dot, "hipMalloc(dot)");
 	if (!e) e |= dalloc((void **)&j->cls, 4 * (size_t)m, "hipMalloc(cls)");
 	if (!e) e |= dalloc((void **)&j->last, 4 * (size_t)m, "hipMalloc(last)");
@@ -178,7 +191,7 @@ void nmfk_job_destroy(nmfk_job *j)
 	if (j->chunk_exec)
 		hipGraphExecDestroy(j->chunk_exec);
 	hipFree(j->V), hipFree(j->Vt), hipFree(j->W), hipFree(j->H), hipFree(j->part);
-	hipFree(j->colW), hipFree(j->colH), hipFree(j->dot), hipFree(j->cls), hipFree(j->last);
+	hipFree(j->colW), hipFree(j->colH), hipFree(j->denW), hipFree(j->dot), hipFree(j->cls), hipFree(j->last);
 	hipFree(j->st);
 	if (j->h_st)
 		hipHostFree(j->h_st);
@@ -198,6 +211,26 @@ int nmfk_job_info_get(const nmfk_job *j, nmfk_job_info *info)
 	info->dV = j->V, info->dVt = j->Vt, info->dW = j->W, info->dH = j->H;
 	info->stream = (void *)j->stream;
 	return 0;
+}
+
+int nmfk_job_set_kernel_path(nmfk_job *j, int path)
+{
+	if (path < NMFK_PATH_AUTO || path > NMFK_PATH_MFMA32) {
+		nmfk_set_error("nmfk_job_set_kernel_path: unknown path %d", path);
+		return NMFK_EINVAL;
+	}
+	nmfk_plan_half_path(j->m, j->n, j->kt, path, &j->planH);
+	nmfk_plan_half_path(j->n, j->m, j->kt, path, &j->planW);
+	if (j->chunk_exec) {
+		hipGraphExecDestroy(j->chunk_exec);
+		j->chunk_exec = NULL;
+	}
+	return 0;
+}
+
+int nmfk_job_kernel_path(const nmfk_job *j)
+{
+	return (int)j->planW.path;
 }
 
 int nmfk_job_sync(nmfk_job *j)
@@ -341,23 +374,25 @@ int nmfk_job_get_H(nmfk_job *j, float *H, size_t ldh)
 
 /* ---------------------------------------------------------------- half-steps */
 
+/* H half: the partial kernel streams V against W (and reduces colsum(W) into denW), the
+ * finish kernel applies the update to H and leaves colsum(H) partials for the W half. */
 int nmfk_enqueue_half_H(nmfk_job *j, const nmfk_state *st)
 {
-	int e = nmfk_launch_partial(j->V, j->ldv, j->H, j->W, j->m, j->n, j->kt, &j->planH, j->part, st,
-				    j->stream);
+	int e = nmfk_launch_partial(j->V, j->ldv, j->H, j->W, j->m, j->n, j->kt, &j->planH, j->part, j->colW,
+				    j->ncolW, j->denW, st, j->stream);
 	if (!e)
-		e = nmfk_launch_finish(j->part, j->planH.S, j->H, j->m, j->k, j->kt, j->colW, j->ncolW, j->colH, 0,
-				       st, j->stream);
+		e = nmfk_launch_finish(j->part, j->planH.S, j->H, j->m, j->k, j->kt, j->denW, j->colH, 0, st,
+				       j->stream);
 	return e;
 }
 
 int nmfk_enqueue_half_W(nmfk_job *j, const nmfk_state *st, int clamp)
 {
-	int e = nmfk_launch_partial(j->Vt, j->ldvt, j->W, j->H, j->n, j->m, j->kt, &j->planW, j->part, st,
-				    j->stream);
+	int e = nmfk_launch_partial(j->Vt, j->ldvt, j->W, j->H, j->n, j->m, j->kt, &j->planW, j->part, j->colH,
+				    j->ncolH, j->denH, st, j->stream);
 	if (!e)
-		e = nmfk_launch_finish(j->part, j->planW.S, j->W, j->n, j->k, j->kt, j->colH, j->ncolH, j->colW,
-				       clamp, st, j->stream);
+		e = nmfk_launch_finish(j->part, j->planW.S, j->W, j->n, j->k, j->kt, j->denH, j->colW, clamp, st,
+				       j->stream);
 	return e;
 }
 
@@ -558,20 +593,20 @@ static int time_loop(nmfk_job *j, uint32_t iters, int kind, double *ms)
 				e = nmfk_enqueue_half_W(j, NULL, 0);
 			break;
 		case 1:
-			e = nmfk_launch_partial(j->V, j->ldv, j->H, j->W, j->m, j->n, j->kt, &j->planH, j->part, NULL,
-						j->stream);
+			e = nmfk_launch_partial(j->V, j->ldv, j->H, j->W, j->m, j->n, j->kt, &j->planH, j->part,
+						j->colW, j->ncolW, j->denW, NULL, j->stream);
 			break;
 		case 2:
-			e = nmfk_launch_finish(j->part, j->planH.S, j->H, j->m, j->k, j->kt, j->colW, j->ncolW,
-					       j->colH, 0, NULL, j->stream);
+			e = nmfk_launch_finish(j->part, j->planH.S, j->H, j->m, j->k, j->kt, j->denW, j->colH, 0,
+					       NULL, j->stream);
 			break;
 		case 3:
 			e = nmfk_launch_partial(j->Vt, j->ldvt, j->W, j->H, j->n, j->m, j->kt, &j->planW, j->part,
-						NULL, j->stream);
+						j->colH, j->ncolH, j->denH, NULL, j->stream);
 			break;
 		case 4:
-			e = nmfk_launch_finish(j->part, j->planW.S, j->W, j->n, j->k, j->kt, j->colH, j->ncolH,
-					       j->colW, 0, NULL, j->stream);
+			e = nmfk_launch_finish(j->part, j->planW.S, j->W, j->n, j->k, j->kt, j->denH, j->colW, 0,
+					       NULL, j->stream);
 			break;
 		}
 	}

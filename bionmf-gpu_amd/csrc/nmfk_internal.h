@@ -21,8 +21,9 @@ struct nmfk_job {
 	size_t ldv, ldvt;
 	float *V, *Vt, *W, *H;         /* device */
 	float *part;                   /* partial contractions, S x p x kt */
-	float *colW, *colH;            /* column-sum partials, one row per 256 rows */
+	float *colW, *colH;            /* column-sum partials, one row per NMFK_FIN_ROWS rows */
 	uint32_t ncolW, ncolH;
+	float *denW, *denH;            /* colsum(W), colsum(H): the half-steps' denominators */
 	double *dot;                   /* FP64 partial sums of the Frobenius / KL kernel */
 	uint32_t ndot;
 	uint32_t *cls, *last;          /* device class vectors */

@@ -36,7 +36,12 @@ typedef struct nmfk_state {
 uint32_t nmfk_kt(uint32_t k);
 
 /* Rows of A whose column sums one partial row covers (k_finish / k_colsum_parts). */
-#define NMFK_FIN_ROWS 256
+#define NMFK_FIN_ROWS 64
+
+/* Kernel paths of the partial kernel. */
+#define NMFK_PATH_AUTO 0     /* VALU for kt <= 8, FP32 MFMA otherwise */
+#define NMFK_PATH_VALU 1     /* FP32 FMA on the vector ALUs */
+#define NMFK_PATH_MFMA32 2   /* FP32 MFMA 16x16x4 (kt % 16 == 0) */
 
 /* Launch plan of one half-step: grid of the partial kernel. */
 typedef struct nmfk_hplan {
@@ -45,36 +50,39 @@ typedef struct nmfk_hplan {
 	uint32_t bchunk;  /* rows of U per b-split */
 	uint32_t ap;      /* a's per lane */
 	uint32_t nw;      /* waves per workgroup */
+	uint32_t path;    /* NMFK_PATH_* actually used */
 } nmfk_hplan;
 
 /* Chooses the partial-kernel grid for updating a p x kt factor against q rows of U. */
 void nmfk_plan_half(uint32_t p, uint32_t q, uint32_t kt, nmfk_hplan *pl);
+void nmfk_plan_half_path(uint32_t p, uint32_t q, uint32_t kt, int path, nmfk_hplan *pl);
 
 /*
  * Half-step, first kernel: part[s][a][c] = sum over b in split s of
  *   B[b][c] * U[b][a] / (sum_c' A[a][c'] * B[b][c'])
  * H half: U = V (q = n, p = m), A = H, B = W.   W half: U = Vt (q = m, p = n), A = W, B = H.
+ * Workgroup (0,0) also writes den[c] = sum_g colB[g][c] (the column sums of B).
  */
 int nmfk_launch_partial(const float *U, size_t ldu, const float *A, const float *B,
 			uint32_t p, uint32_t q, uint32_t kt, const nmfk_hplan *pl,
-			float *part, const nmfk_state *st, hipStream_t stream);
+			float *part, const float *colB, uint32_t ncolB, float *den,
+			const nmfk_state *st, hipStream_t stream);
 
 /*
  * Half-step, second kernel:
- *   A[a][c] = (A[a][c] * sum_s part[s][a][c]) / sum_g colB[g][c]     for c < k
+ *   A[a][c] = (A[a][c] * sum_s part[s][a][c]) / den[c]     for c < k
  * then optionally the R_MIN clamp, then colA[blk][c] = column sums of the new A over
  * each block of NMFK_FIN_ROWS rows (the next half-step's denominators).
  */
 int nmfk_launch_finish(const float *part, uint32_t S, float *A, uint32_t p, uint32_t k, uint32_t kt,
-		       const float *colB, uint32_t ncolB, float *colA, int clamp,
-		       const nmfk_state *st, hipStream_t stream);
+		       const float *den, float *colA, int clamp, const nmfk_state *st,
+		       hipStream_t stream);
 
 /* colA[blk][c] = sum of A[a][c] over the rows of block blk (same order as k_finish). */
 int nmfk_launch_colsum_parts(const float *A, uint32_t p, uint32_t kt, float *colA, hipStream_t stream);
 
-/* out[c] = sum_g parts[g][c], g ascending. */
-int nmfk_launch_sum_parts(const float *parts, uint32_t nparts, uint32_t kt, uint32_t width,
-			  float *out, hipStream_t stream);
+/* den[c] = sum_g parts[g][c] in the fixed order the partial kernel uses. */
+int nmfk_launch_den(const float *parts, uint32_t nparts, uint32_t kt, float *den, hipStream_t stream);
 
 /*
  * Convergence test (src/NMF_GPU.c:611-708): clamps H (m x k, pitch kt) in place, writes

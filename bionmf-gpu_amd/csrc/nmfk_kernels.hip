@@ -106,70 +106,68 @@ __device__ __forceinline__ double wave_sum_d(double v)
 	return v;
 }
 
-// Sum of column c over nparts partial rows, g ascending (used everywhere such sums appear).
-__device__ __forceinline__ float sum_parts_col(const float *__restrict__ parts, int nparts, int ld, int c)
-{
-	float s = 0.0f;
-	for (int g = 0; g < nparts; g++)
-		s += parts[(size_t)g * ld + c];
-	return s;
-}
-
-// Column sums of a block of 256 rows (one row per thread, x[] = that row), written as one
-// partial row.  Shared by k_finish and k_colsum_parts so both give identical bits.
+// Denominator of a half-step: den[c] = sum over the nparts column-sum partials of the
+// other factor.  Four interleaved ascending partial sums per column, combined in a fixed
+// order, so every kernel that forms it gets the same bits.  Needs 4*KT <= blockDim.x.
 template <int KT>
-__device__ __forceinline__ void block_colsum_store(float (&x)[KT], float *__restrict__ out)
+__device__ __forceinline__ void den_reduce(const float *__restrict__ parts, int nparts, float *__restrict__ den)
 {
-	__shared__ float wsum[4][KT];
-	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-	for (int c = 0; c < KT; c++) {
-		const float s = wave_sum(x[c]);
-		if (lane == 0)
-			wsum[wave][c] = s;
+	__shared__ float s4[4 * KT];
+	const int t = threadIdx.x;
+	if (t < 4 * KT) {
+		const int c = t >> 2, j = t & 3;
+		float s = 0.0f;
+		for (int g = j; g < nparts; g += 4)
+			s += parts[(size_t)g * KT + c];
+		s4[t] = s;
 	}
 	__syncthreads();
-	for (int c = threadIdx.x; c < KT; c += 256)
-		out[c] = ((wsum[0][c] + wsum[1][c]) + wsum[2][c]) + wsum[3][c];
+	if (t < KT)
+		den[t] = ((s4[4 * t] + s4[4 * t + 1]) + s4[4 * t + 2]) + s4[4 * t + 3];
+}
+
+// One finish block = NMFK_FIN_ROWS (64) rows of A, four threads per row, each thread
+// KT/4 consecutive columns.  Column sums of the block's rows (ascending rows) are written
+// as one partial row; shared by k_finish and k_colsum_parts so both give the same bits.
+template <int KT>
+__device__ __forceinline__ void block_colsum_store(const float (&x)[KT / 4], float *__restrict__ out)
+{
+	__shared__ float tile[NMFK_FIN_ROWS][KT + 1];
+	const int row = threadIdx.x >> 2, q = threadIdx.x & 3;
+#pragma unroll
+	for (int i = 0; i < KT / 4; i++)
+		tile[row][q * (KT / 4) + i] = x[i];
+	__syncthreads();
+	if (threadIdx.x < KT) {
+		float s = 0.0f;
+		for (int r = 0; r < NMFK_FIN_ROWS; r++)
+			s += tile[r][threadIdx.x];
+		out[threadIdx.x] = s;
+	}
 }
 
 // --------------------------------------------------------------------------------------
 // Half-step kernels.
 // --------------------------------------------------------------------------------------
 
+// Vector load of AP consecutive U values.  Every lane reads in bounds: rows are padded
+// to a multiple of 4 (so a0 + AP <= ldu whenever a0 < p) and lanes past p read column 0;
+// values past p are then zeroed with selects (no branches in the streaming loop).
 template <int AP>
-struct vecf;
-template <>
-struct vecf<1> {
-	typedef float type;
-};
-template <>
-struct vecf<2> {
-	typedef float2 type;
-};
-template <>
-struct vecf<4> {
-	typedef float4 type;
-};
-
-template <int AP>
-__device__ __forceinline__ void load_u(const float *__restrict__ src, bool full, int avail, float (&u)[AP])
+__device__ __forceinline__ void load_u(const float *__restrict__ src, int avail, float (&u)[AP])
 {
-	if (full) {
-		if constexpr (AP == 4) {
-			const float4 t = *reinterpret_cast<const float4 *>(src);
-			u[0] = t.x, u[1] = t.y, u[2] = t.z, u[3] = t.w;
-		} else if constexpr (AP == 2) {
-			const float2 t = *reinterpret_cast<const float2 *>(src);
-			u[0] = t.x, u[1] = t.y;
-		} else {
-			u[0] = src[0];
-		}
+	if constexpr (AP == 4) {
+		const float4 t = *reinterpret_cast<const float4 *>(src);
+		u[0] = t.x, u[1] = t.y, u[2] = t.z, u[3] = t.w;
+	} else if constexpr (AP == 2) {
+		const float2 t = *reinterpret_cast<const float2 *>(src);
+		u[0] = t.x, u[1] = t.y;
 	} else {
-#pragma unroll
-		for (int r = 0; r < AP; r++)
-			u[r] = (r < avail) ? src[r] : 0.0f;
+		u[0] = src[0];
 	}
+#pragma unroll
+	for (int r = 0; r < AP; r++)
+		u[r] = (r < avail) ? u[r] : 0.0f;
 }
 
 // part[blockIdx.y][a][c] = sum_{b in split} B[b][c] * U[b][a] / (A[a] . B[b])
@@ -178,34 +176,51 @@ __global__ __launch_bounds__(64 * NW) void k_partial(const float *__restrict__ U
 						     const float *__restrict__ A,
 						     const float *__restrict__ B, int p, int q,
 						     int bchunk, float *__restrict__ part,
+						     const float *__restrict__ colB, int ncolB,
+						     float *__restrict__ den,
 						     const nmfk_state *__restrict__ st)
 {
 	if (st != nullptr && st->converged)
 		return;
+	// Workgroup (0,0) also reduces the other factor's column-sum partials into the
+	// denominators the finish kernel divides by.
+	if (blockIdx.x == 0 && blockIdx.y == 0)
+		den_reduce<KT>(colB, ncolB, den);
 	const int lane = threadIdx.x & 63;
 	const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
 	const int a0 = (blockIdx.x * 64 + lane) * AP;
 	const int avail = p - a0;
-	const bool full = avail >= AP;
 
 	float av[AP][KT], acc[AP][KT];
 #pragma unroll
 	for (int r = 0; r < AP; r++) {
 		const bool ok = r < avail;
+		const float *Ar = A + (size_t)(ok ? a0 + r : 0) * KT;
 #pragma unroll
 		for (int c = 0; c < KT; c++) {
-			av[r][c] = ok ? A[(size_t)(a0 + r) * KT + c] : 0.0f;
+			av[r][c] = ok ? Ar[c] : 0.0f;
 			acc[r][c] = 0.0f;
 		}
 	}
 
 	const int b_begin = blockIdx.y * bchunk;
 	const int b_end = min(q, b_begin + bchunk);
-	const float *Ucol = U + a0;
-#pragma unroll 2
-	for (int b = b_begin + wave; b < b_end; b += NW) {
+	const float *Ucol = U + (avail > 0 ? a0 : 0);
+	// One row of U is prefetched ahead so its latency overlaps the current row's math.
+	int b = b_begin + wave;
+	float un[AP];
+#pragma unroll
+	for (int r = 0; r < AP; r++)
+		un[r] = 0.0f;
+	if (b < b_end)
+		load_u<AP>(Ucol + (size_t)b * ldu, avail, un);
+	for (; b < b_end; b += NW) {
 		float u[AP];
-		load_u<AP>(Ucol + (size_t)b * ldu, full, avail, u);
+#pragma unroll
+		for (int r = 0; r < AP; r++)
+			u[r] = un[r];
+		if (b + NW < b_end)
+			load_u<AP>(Ucol + (size_t)(b + NW) * ldu, avail, un);
 		const float *Bb = B + (size_t)b * KT;
 		float bv[KT];
 #pragma unroll
@@ -217,7 +232,10 @@ __global__ __launch_bounds__(64 * NW) void k_partial(const float *__restrict__ U
 #pragma unroll
 			for (int c = 0; c < KT; c++)
 				wh = fmaf(av[r][c], bv[c], wh);
-			const float ratio = u[r] / wh;
+			// ratio = U / WH with the hardware reciprocal (<= 1 ulp) and one rounding of the
+			// product, the fast division the reference's default build uses
+			// (include/real_type.h:283-288, --use_fast_math).
+			const float ratio = u[r] * __builtin_amdgcn_rcpf(wh);
 #pragma unroll
 			for (int c = 0; c < KT; c++)
 				acc[r][c] = fmaf(ratio, bv[c], acc[r][c]);
@@ -262,58 +280,211 @@ __global__ __launch_bounds__(64 * NW) void k_partial(const float *__restrict__ U
 	}
 }
 
-// A[a][c] = (A[a][c] * sum_s part[s][a][c]) / sum_g colB[g][c], optional clamp, then the
-// column sums of the new A per block of 256 rows.  (kernel_mul_div: multiply first, then
-// divide, src/GPU_kernels.cu:2029-2033.)
+// --------------------------------------------------------------------------------------
+// The same half-step on the matrix cores (KT a multiple of 16), FP32 MFMA 16x16x4.
+//
+// A wave owns 64 rows a of the factor being updated as 4 tiles t of 16, row
+// a = a0 + 4*j + t for lane column j = lane % 16 (so each lane's 4 values of one U row are
+// one float4), and walks 16-row blocks of U.  Lane group g = lane / 16.  Per block:
+//   GEMM1  S[b][a] = sum_c B[b][c] A[a][c]   M = 16 rows b, N = 16 a's of tile t, K = c;
+//          the output lands as lane (g, j) holding S[bb + 4g + r][a0 + 4j + t], r = 0..3;
+//   ratio  R = U / S in registers (reciprocal, as the VALU kernel);
+//   GEMM2  acc[a][c] += sum_b R[b][a] B[b][c]  M = 16 a's, N = 16 c's, K = b: GEMM1's
+//          output register r is used directly as the A operand of the K-chunk made of rows
+//          {bb + 4g + r : g = 0..3}; the B operand is B at those same rows.
+// In both products the summed index is assigned to lane groups identically on the two
+// operands, which is all a contraction needs, so no data moves between them.
+// --------------------------------------------------------------------------------------
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+template <int KT, int NW>
+__global__ __launch_bounds__(64 * NW) void k_partial_mfma(const float *__restrict__ U, size_t ldu,
+							  const float *__restrict__ A,
+							  const float *__restrict__ B, int p, int q,
+							  int bchunk, float *__restrict__ part,
+							  const float *__restrict__ colB, int ncolB,
+							  float *__restrict__ den,
+							  const nmfk_state *__restrict__ st)
+{
+	static_assert(KT % 16 == 0, "MFMA path needs KT % 16 == 0");
+	constexpr int H = KT / 16;     // 16-column groups of the rank
+	if (st != nullptr && st->converged)
+		return;
+	if (blockIdx.x == 0 && blockIdx.y == 0)
+		den_reduce<KT>(colB, ncolB, den);
+
+	const int lane = threadIdx.x & 63;
+	const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+	const int j = lane & 15, g = lane >> 4;
+	const int a0 = blockIdx.x * 64;
+	const int aj = a0 + 4 * j;               // first of this lane's 4 rows a (one per tile)
+	const int avail = p - aj;                // tiles t < avail are real rows
+
+	// GEMM1 B operand: lane (g, j) supplies A[aj + t][16h + 4g + qq] for K-chunk (h, qq).
+	floatx4 aop[4][H];
+#pragma unroll
+	for (int t = 0; t < 4; t++) {
+		const bool ok = t < avail;
+		const float *Ar = A + (size_t)(ok ? aj + t : 0) * KT + 4 * g;
+#pragma unroll
+		for (int h = 0; h < H; h++) {
+			const float4 v = *reinterpret_cast<const float4 *>(Ar + 16 * h);
+			aop[t][h] = ok ? floatx4{v.x, v.y, v.z, v.w} : floatx4{0.f, 0.f, 0.f, 0.f};
+		}
+	}
+	floatx4 acc[4][H];
+#pragma unroll
+	for (int t = 0; t < 4; t++)
+#pragma unroll
+		for (int h = 0; h < H; h++)
+			acc[t][h] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+	const int b_begin = blockIdx.y * bchunk;
+	const int b_end = min(q, b_begin + bchunk);
+	const float *Ucol = U + (avail > 0 ? aj : 0);
+
+	// U rows bb + 4g + r, columns aj..aj+3, for the next block (prefetched).
+	auto load_block = [&](int bb, float4 (&u)[4]) {
+#pragma unroll
+		for (int r = 0; r < 4; r++) {
+			const int b = bb + 4 * g + r;
+			if (b < b_end)
+				u[r] = *reinterpret_cast<const float4 *>(Ucol + (size_t)b * ldu);
+			else
+				u[r] = make_float4(0.f, 0.f, 0.f, 0.f);
+		}
+	};
+	float4 un[4];
+	int bb = b_begin + 16 * wave;
+	if (bb < b_end)
+		load_block(bb, un);
+	for (; bb < b_end; bb += 16 * NW) {
+		float4 u[4];
+#pragma unroll
+		for (int r = 0; r < 4; r++)
+			u[r] = un[r];
+		if (bb + 16 * NW < b_end)
+			load_block(bb + 16 * NW, un);
+
+		// GEMM1 A operand: lane (g, j) supplies B[bb + j][16h + 4g + qq].
+		floatx4 bop1[H];
+		{
+			const bool ok = bb + j < b_end;
+			const float *Br = B + (size_t)(ok ? bb + j : 0) * KT + 4 * g;
+#pragma unroll
+			for (int h = 0; h < H; h++) {
+				const float4 v = *reinterpret_cast<const float4 *>(Br + 16 * h);
+				bop1[h] = ok ? floatx4{v.x, v.y, v.z, v.w} : floatx4{0.f, 0.f, 0.f, 0.f};
+			}
+		}
+		// GEMM2 B operand: lane (g, j) supplies B[bb + 4g + r][16h + j].
+		float bop2[4][H];
+#pragma unroll
+		for (int r = 0; r < 4; r++) {
+			const int b = bb + 4 * g + r;
+			const bool ok = b < b_end;
+			const float *Br = B + (size_t)(ok ? b : 0) * KT + j;
+#pragma unroll
+			for (int h = 0; h < H; h++)
+				bop2[r][h] = ok ? Br[16 * h] : 0.0f;
+		}
+
+#pragma unroll
+		for (int t = 0; t < 4; t++) {
+			floatx4 s = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+			for (int h = 0; h < H; h++)
+#pragma unroll
+				for (int qq = 0; qq < 4; qq++)
+					s = __builtin_amdgcn_mfma_f32_16x16x4f32(bop1[h][qq], aop[t][h][qq], s, 0, 0, 0);
+			float rt[4];
+#pragma unroll
+			for (int r = 0; r < 4; r++) {
+				const float uv = (t == 0) ? u[r].x : (t == 1) ? u[r].y : (t == 2) ? u[r].z : u[r].w;
+				const bool ok = (t < avail) && (bb + 4 * g + r < b_end);
+				rt[r] = ok ? uv * __builtin_amdgcn_rcpf(s[r]) : 0.0f;
+			}
+#pragma unroll
+			for (int h = 0; h < H; h++)
+#pragma unroll
+				for (int r = 0; r < 4; r++)
+					acc[t][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(rt[r], bop2[r][h], acc[t][h], 0, 0, 0);
+		}
+	}
+
+	// acc[t][h][r] = partial for a = a0 + 4*(4g + r) + t, c = 16h + j.  Sum the NW waves in a
+	// fixed order through LDS, then write the 64 x KT block row-major.
+	__shared__ float red[NW][4][H][4][64];
+#pragma unroll
+	for (int t = 0; t < 4; t++)
+#pragma unroll
+		for (int h = 0; h < H; h++)
+#pragma unroll
+			for (int r = 0; r < 4; r++)
+				red[wave][t][h][r][lane] = acc[t][h][r];
+	__syncthreads();
+	float *dst = part + (size_t)blockIdx.y * p * KT;
+	for (int e = threadIdx.x; e < 64 * KT; e += 64 * NW) {
+		const int al = e / KT, c = e % KT;
+		const int t = al & 3, r = (al >> 2) & 3, gg = al >> 4;
+		const int h = c >> 4, ln = 16 * gg + (c & 15);
+		float v = red[0][t][h][r][ln];
+#pragma unroll
+		for (int w = 1; w < NW; w++)
+			v += red[w][t][h][r][ln];
+		if (a0 + al < p)
+			dst[(size_t)(a0 + al) * KT + c] = v;
+	}
+}
+
+// A[a][c] = (A[a][c] * sum_s part[s][a][c]) / den[c], optional clamp, then the column sums
+// of the new A per block of 64 rows (the next half-step's denominators).  Kernel_mul_div
+// order: multiply first, then divide (src/GPU_kernels.cu:2029-2033).
 template <int KT>
 __global__ __launch_bounds__(256) void k_finish(const float *__restrict__ part, int S,
 						float *__restrict__ A, int p, int k,
-						const float *__restrict__ colB, int ncolB,
-						float *__restrict__ colA, int clamp,
-						const nmfk_state *__restrict__ st)
+						const float *__restrict__ den, float *__restrict__ colA,
+						int clamp, const nmfk_state *__restrict__ st)
 {
 	if (st != nullptr && st->converged)
 		return;
-	__shared__ float den[KT];
-	for (int c = threadIdx.x; c < KT; c += 256)
-		den[c] = sum_parts_col(colB, ncolB, KT, c);
-	__syncthreads();
-
-	const int a = blockIdx.x * 256 + threadIdx.x;
-	float x[KT];
+	constexpr int Q = KT / 4;
+	const int a = blockIdx.x * NMFK_FIN_ROWS + (threadIdx.x >> 2);
+	const int c0 = (threadIdx.x & 3) * Q;
+	float x[Q];
 	if (a < p) {
 		const size_t pst = (size_t)p * KT;
-		const float *pa = part + (size_t)a * KT;
-		float s[KT];
+		const float *pa = part + (size_t)a * KT + c0;
+		float s[Q];
 #pragma unroll
-		for (int c = 0; c < KT; c++)
-			s[c] = pa[c];
+		for (int i = 0; i < Q; i++)
+			s[i] = pa[i];
 		for (int sp = 1; sp < S; sp++) {
 			const float *ps = pa + sp * pst;
 #pragma unroll
-			for (int c = 0; c < KT; c++)
-				s[c] += ps[c];
+			for (int i = 0; i < Q; i++)
+				s[i] += ps[i];
 		}
-		float *Ar = A + (size_t)a * KT;
+		float *Ar = A + (size_t)a * KT + c0;
 #pragma unroll
-		for (int c = 0; c < KT; c++) {
-			if (c < k) {
-				float v = Ar[c] * s[c];
-				v = v / den[c];
+		for (int i = 0; i < Q; i++) {
+			float v = 0.0f;
+			if (c0 + i < k) {
+				v = Ar[i] * s[i];
+				v = v / den[c0 + i];
 				if (clamp)
 					v = clamp_rmin(v);
-				x[c] = v;
-			} else {
-				x[c] = 0.0f;
 			}
+			x[i] = v;
 		}
 #pragma unroll
-		for (int c = 0; c < KT; c += 4)
-			*reinterpret_cast<float4 *>(Ar + c) = make_float4(x[c], x[c + 1], x[c + 2], x[c + 3]);
+		for (int i = 0; i < Q; i++)
+			Ar[i] = x[i];
 	} else {
 #pragma unroll
-		for (int c = 0; c < KT; c++)
-			x[c] = 0.0f;
+		for (int i = 0; i < Q; i++)
+			x[i] = 0.0f;
 	}
 	block_colsum_store<KT>(x, colA + (size_t)blockIdx.x * KT);
 }
@@ -322,20 +493,20 @@ template <int KT>
 __global__ __launch_bounds__(256) void k_colsum_parts(const float *__restrict__ A, int p,
 						      float *__restrict__ colA)
 {
-	const int a = blockIdx.x * 256 + threadIdx.x;
-	float x[KT];
+	constexpr int Q = KT / 4;
+	const int a = blockIdx.x * NMFK_FIN_ROWS + (threadIdx.x >> 2);
+	const int c0 = (threadIdx.x & 3) * Q;
+	float x[Q];
 #pragma unroll
-	for (int c = 0; c < KT; c++)
-		x[c] = (a < p) ? A[(size_t)a * KT + c] : 0.0f;
+	for (int i = 0; i < Q; i++)
+		x[i] = (a < p) ? A[(size_t)a * KT + c0 + i] : 0.0f;
 	block_colsum_store<KT>(x, colA + (size_t)blockIdx.x * KT);
 }
 
-__global__ void k_sum_parts(const float *__restrict__ parts, int nparts, int ld, int width,
-			    float *__restrict__ out)
+template <int KT>
+__global__ __launch_bounds__(256) void k_den(const float *__restrict__ parts, int nparts, float *__restrict__ den)
 {
-	const int c = blockIdx.x * blockDim.x + threadIdx.x;
-	if (c < width)
-		out[c] = sum_parts_col(parts, nparts, ld, c);
+	den_reduce<KT>(parts, nparts, den);
 }
 
 // --------------------------------------------------------------------------------------
@@ -714,13 +885,16 @@ static inline unsigned grid1d(size_t total, unsigned block)
 	return (unsigned)(g ? g : 1);
 }
 
-#define NMFK_KT_LIST(X) X(4) X(8) X(12) X(16) X(20) X(24) X(28) X(32) X(40) X(48) X(56) X(64)
+#define NMFK_KT_LIST(X) X(4) X(8) X(16) X(32) X(48) X(64)
+#define NMFK_KT_MFMA_LIST(X) X(16) X(32) X(48) X(64)
 
 extern "C" uint32_t nmfk_kt(uint32_t k)
 {
-	if (k <= 32)
+	// k <= 8 stays on the VALU kernel (memory-bound there); larger ranks are padded to the
+	// 16-column granularity of the MFMA kernel.
+	if (k <= 8)
 		return (k + 3u) & ~3u;
-	return (k + 7u) & ~7u;
+	return (k + 15u) & ~15u;
 }
 
 static inline uint32_t default_ap(uint32_t kt)
@@ -728,51 +902,91 @@ static inline uint32_t default_ap(uint32_t kt)
 	return kt <= 8 ? 4u : (kt <= 32 ? 2u : 1u);
 }
 
-extern "C" void nmfk_plan_half(uint32_t p, uint32_t q, uint32_t kt, nmfk_hplan *pl)
+extern "C" void nmfk_plan_half_path(uint32_t p, uint32_t q, uint32_t kt, int path, nmfk_hplan *pl)
 {
-	const uint32_t ap = default_ap(kt);
-	const uint32_t nw = 8;
+	if (path == NMFK_PATH_AUTO)
+		path = (kt % 16 == 0) ? NMFK_PATH_MFMA32 : NMFK_PATH_VALU;
+	if (path != NMFK_PATH_VALU && kt % 16 != 0)
+		path = NMFK_PATH_VALU;
+	const uint32_t ap = path == NMFK_PATH_VALU ? default_ap(kt) : 1u;  // MFMA: 64 rows a per workgroup
+	const uint32_t nw = path == NMFK_PATH_VALU ? 8u : 4u;
+	const uint32_t rows_per_step = path == NMFK_PATH_VALU ? 1u : 16u;  // rows of U per wave step
 	const uint32_t nab = (p + 64u * ap - 1u) / (64u * ap);
-	const uint32_t target = 512;  // workgroups of 8 waves: 2 per CU on 256 CUs
+	const uint32_t target = path == NMFK_PATH_VALU ? 512u : 512u;   // workgroups: 2 per CU on 256 CUs
 	uint32_t S = (target + nab - 1u) / nab;
-	const uint32_t smax = (q + nw * 8u - 1u) / (nw * 8u);  // at least 8 rows of U per wave
+	const uint32_t min_rows = nw * rows_per_step * 2u;  // at least two steps per wave
+	const uint32_t smax = (q + min_rows - 1u) / min_rows;
 	if (S > smax)
 		S = smax;
 	if (S < 1)
 		S = 1;
 	uint32_t bchunk = (q + S - 1u) / S;
+	bchunk = (bchunk + rows_per_step - 1u) / rows_per_step * rows_per_step;
 	S = (q + bchunk - 1u) / bchunk;
 	pl->nab = nab;
 	pl->S = S;
 	pl->bchunk = bchunk;
 	pl->ap = ap;
 	pl->nw = nw;
+	pl->path = (uint32_t)path;
+}
+
+extern "C" void nmfk_plan_half(uint32_t p, uint32_t q, uint32_t kt, nmfk_hplan *pl)
+{
+	nmfk_plan_half_path(p, q, kt, NMFK_PATH_AUTO, pl);
 }
 
 template <int KT, int AP>
 static int launch_partial_t(const float *U, size_t ldu, const float *A, const float *B, uint32_t p,
-			    uint32_t q, const nmfk_hplan *pl, float *part, const nmfk_state *st,
-			    hipStream_t stream)
+			    uint32_t q, const nmfk_hplan *pl, float *part, const float *colB, uint32_t ncolB,
+			    float *den, const nmfk_state *st, hipStream_t stream)
 {
 	dim3 grid(pl->nab, pl->S);
 	if (pl->nw != 8)
 		return (int)hipErrorInvalidValue;
 	hipLaunchKernelGGL((k_partial<KT, AP, 8>), grid, dim3(512), 0, stream, U, ldu, A, B, (int)p,
-			   (int)q, (int)pl->bchunk, part, st);
+			   (int)q, (int)pl->bchunk, part, colB, (int)ncolB, den, st);
+	return (int)hipGetLastError();
+}
+
+template <int KT>
+static int launch_partial_mfma_t(const float *U, size_t ldu, const float *A, const float *B, uint32_t p,
+				 uint32_t q, const nmfk_hplan *pl, float *part, const float *colB, uint32_t ncolB,
+				 float *den, const nmfk_state *st, hipStream_t stream)
+{
+	dim3 grid(pl->nab, pl->S);
+	if (pl->nw != 4 || pl->bchunk % 16 != 0)
+		return (int)hipErrorInvalidValue;
+	hipLaunchKernelGGL((k_partial_mfma<KT, 4>), grid, dim3(256), 0, stream, U, ldu, A, B, (int)p, (int)q,
+			   (int)pl->bchunk, part, colB, (int)ncolB, den, st);
 	return (int)hipGetLastError();
 }
 
 extern "C" int nmfk_launch_partial(const float *U, size_t ldu, const float *A, const float *B, uint32_t p,
 				   uint32_t q, uint32_t kt, const nmfk_hplan *pl, float *part,
+				   const float *colB, uint32_t ncolB, float *den,
 				   const nmfk_state *st, hipStream_t stream)
 {
-	if (ldu % 4 != 0 || pl->ap != default_ap(kt))
+	if (ldu % 4 != 0)
+		return (int)hipErrorInvalidValue;
+	if (pl->path == NMFK_PATH_MFMA32) {
+		switch (kt) {
+#define X(K)                                                                                          \
+	case K:                                                                                       \
+		return launch_partial_mfma_t<K>(U, ldu, A, B, p, q, pl, part, colB, ncolB, den, st, stream);
+			NMFK_KT_MFMA_LIST(X)
+#undef X
+		default:
+			return (int)hipErrorInvalidValue;
+		}
+	}
+	if (pl->ap != default_ap(kt))
 		return (int)hipErrorInvalidValue;
 	switch (kt) {
 #define X(K)                                                                                          \
 	case K:                                                                                       \
 		return launch_partial_t<K, (K <= 8 ? 4 : (K <= 32 ? 2 : 1))>(U, ldu, A, B, p, q, pl, part, \
-									      st, stream);
+									      colB, ncolB, den, st, stream);
 		NMFK_KT_LIST(X)
 #undef X
 	default:
@@ -781,15 +995,15 @@ extern "C" int nmfk_launch_partial(const float *U, size_t ldu, const float *A, c
 }
 
 extern "C" int nmfk_launch_finish(const float *part, uint32_t S, float *A, uint32_t p, uint32_t k, uint32_t kt,
-				  const float *colB, uint32_t ncolB, float *colA, int clamp,
-				  const nmfk_state *st, hipStream_t stream)
+				  const float *den, float *colA, int clamp, const nmfk_state *st,
+				  hipStream_t stream)
 {
 	const unsigned g = (p + NMFK_FIN_ROWS - 1) / NMFK_FIN_ROWS;
 	switch (kt) {
 #define X(K)                                                                                           \
 	case K:                                                                                        \
 		hipLaunchKernelGGL((k_finish<K>), dim3(g), dim3(256), 0, stream, part, (int)S, A, (int)p, \
-				   (int)k, colB, (int)ncolB, colA, clamp, st);                          \
+				   (int)k, den, colA, clamp, st);                                       \
 		return (int)hipGetLastError();
 		NMFK_KT_LIST(X)
 #undef X
@@ -813,12 +1027,18 @@ extern "C" int nmfk_launch_colsum_parts(const float *A, uint32_t p, uint32_t kt,
 	}
 }
 
-extern "C" int nmfk_launch_sum_parts(const float *parts, uint32_t nparts, uint32_t kt, uint32_t width, float *out,
-				     hipStream_t stream)
+extern "C" int nmfk_launch_den(const float *parts, uint32_t nparts, uint32_t kt, float *den, hipStream_t stream)
 {
-	hipLaunchKernelGGL(k_sum_parts, dim3((width + 63) / 64), dim3(64), 0, stream, parts, (int)nparts,
-			   (int)kt, (int)width, out);
-	return (int)hipGetLastError();
+	switch (kt) {
+#define X(K)                                                                                      \
+	case K:                                                                                   \
+		hipLaunchKernelGGL((k_den<K>), dim3(1), dim3(256), 0, stream, parts, (int)nparts, den);  \
+		return (int)hipGetLastError();
+		NMFK_KT_LIST(X)
+#undef X
+	default:
+		return (int)hipErrorInvalidValue;
+	}
 }
 
 extern "C" int nmfk_launch_test(float *H, uint32_t m, uint32_t k, uint32_t kt, uint32_t *cls, uint32_t *last,

@@ -94,6 +94,11 @@ int nmfk_job_dot_kl(nmfk_job *job, double *dot_V, double *dot_VWH, double *kl);
 /* Synchronizes the job's stream. */
 int nmfk_job_sync(nmfk_job *job);
 
+/* Kernel path of the half-step contractions: 0 = automatic (vector ALUs for k <= 8, FP32
+ * matrix cores above), 1 = vector ALUs, 2 = FP32 matrix cores (k padded to 16). */
+int nmfk_job_set_kernel_path(nmfk_job *job, int path);
+int nmfk_job_kernel_path(const nmfk_job *job);
+
 /* Raw device pointers and shape (for the drop-in layer and for measurements). */
 typedef struct nmfk_job_info {
 	uint32_t n, m, k, kt;

@@ -43,11 +43,15 @@ def test_V_roundtrip_and_generators(nmfk, orc):
         assert rel_err(job.get_V(), orc.gen_planted(n, m, 6, 4, 0.3)) < 1e-6
 
 
+@pytest.mark.parametrize("path", [0, 1, 2])
 @pytest.mark.parametrize("k", KS)
-def test_half_steps_match_oracle(nmfk, orc, k):
+def test_half_steps_match_oracle(nmfk, orc, k, path):
+    # path 0: automatic (vector ALUs for k <= 8, FP32 matrix cores above), 1: vector ALUs,
+    # 2: FP32 matrix cores (falls back to the vector ALUs where k <= 8).
     n, m = 700, 150
     job, V = make_job(nmfk, orc, n, m, k, seed_v=k, seed_init=k + 100)
     with job:
+        job.set_kernel_path(path)
         W0, H0 = orc.init_WH(n, m, k, k + 100)
         job.update_H()
         H1 = orc.update_H(V, W0, H0)
@@ -57,7 +61,8 @@ def test_half_steps_match_oracle(nmfk, orc, k):
         assert rel_err(job.get_W(), W1) < 1e-5
 
 
-@pytest.mark.parametrize("n,m,k", [(1, 1, 1), (5, 3, 2), (63, 65, 3), (1000, 7, 7), (9, 2000, 2)])
+@pytest.mark.parametrize("n,m,k", [(1, 1, 1), (5, 3, 2), (63, 65, 3), (1000, 7, 7), (9, 2000, 2),
+                                   (17, 33, 9), (100, 13, 13), (31, 1100, 20), (2000, 47, 40)])
 def test_half_steps_ragged_shapes(nmfk, orc, n, m, k):
     job, V = make_job(nmfk, orc, n, m, k, seed_v=n + m, seed_init=k)
     with job:
@@ -122,6 +127,23 @@ def test_run_equals_reference_structured_steps(nmfk, orc):
             b.update_W()
         assert np.array_equal(a.get_W(), b.get_W())
         assert np.array_equal(a.get_H(), b.get_H())
+
+
+def test_matrix_core_and_vector_paths_agree(nmfk, orc):
+    n, m, k = 2500, 300, 16
+    V = orc.gen_lognormal(n, m, 8, 0.0, 1.0)
+    res = []
+    for path in (1, 2):
+        with nmfk.Job(n, m, k) as j:
+            j.set_V(V)
+            j.init_random(4)
+            j.set_kernel_path(path)
+            assert j.kernel_path() == path
+            r = j.run(200, 10, 5)
+            res.append((r, j.get_classification(), j.get_W(), j.get_H()))
+    (r1, c1, W1, H1), (r2, c2, W2, H2) = res
+    assert r1["iterations"] == r2["iterations"] and np.array_equal(c1, c2)
+    assert rel_err(W2, W1) < 1e-3 and rel_err(H2, H1) < 1e-3
 
 
 def test_runs_are_deterministic(nmfk, orc):

@@ -111,6 +111,9 @@ def load(path: os.PathLike | str | None = None):
     _sig(lib, "nmfk_job_time_iterations", i32, [vp, u32, C.POINTER(Timing)])
     _sig(lib, "nmfk_job_set_kernel_path", i32, [vp, i32])
     _sig(lib, "nmfk_job_kernel_path", i32, [vp])
+    _sig(lib, "nmfk_tune_set", i32, [i32, i32])
+    _sig(lib, "nmfk_tune_get", i32, [i32])
+    _sig(lib, "nmfk_measure_stream", i32, [sz, u32, u32, C.POINTER(dbl), C.POINTER(dbl)])
     _lib = lib
     return lib
 
@@ -136,6 +139,25 @@ def gpu_count() -> int:
     n = C.c_int(0)
     rc = hip.hipGetDeviceCount(C.byref(n))
     return n.value if rc == 0 else 0
+
+
+TUNE_WGS_VALU, TUNE_WGS_MFMA, TUNE_MIN_STEPS, TUNE_MAX_SPLITS, TUNE_MFMA_NA, TUNE_MFMA_NB = 0, 1, 2, 3, 4, 5
+
+
+def tune_set(key: int, value: int):
+    """Planner tunables (nmfk_tune_set): apply to jobs created afterwards."""
+    _check(load().nmfk_tune_set(key, value), "tune_set")
+
+
+def tune_get(key: int) -> int:
+    return load().nmfk_tune_get(key)
+
+
+def measure_stream(nbytes: int = 1 << 30, iters: int = 10, blocks: int = 4096):
+    """Measured read and copy bandwidth (GB/s) of plain streaming kernels."""
+    r, c = C.c_double(), C.c_double()
+    _check(load().nmfk_measure_stream(nbytes, iters, blocks, C.byref(r), C.byref(c)), "measure_stream")
+    return r.value, c.value
 
 
 def default_options(k: int = 2, nIters: int = 2000, niter_test_conv: int = 10,

@@ -141,6 +141,7 @@ nmfk_job *nmfk_job_create(uint32_t n, uint32_t m, uint32_t k)
 	j->n = n, j->m = m, j->k = k, j->kt = L.kt;
 	j->ldv = L.ldv, j->ldvt = L.ldvt;
 	j->planH = L.planH, j->planW = L.planW;
+	j->part_cap = L.n_part;
 	j->ncolW = (n + NMFK_FIN_ROWS - 1) / NMFK_FIN_ROWS;
 	j->ncolH = (m + NMFK_FIN_ROWS - 1) / NMFK_FIN_ROWS;
 	j->ndot = nmfk_dot_kl_blocks(n, m);
@@ -219,8 +220,16 @@ int nmfk_job_set_kernel_path(nmfk_job *j, int path)
 		nmfk_set_error("nmfk_job_set_kernel_path: unknown path %d", path);
 		return NMFK_EINVAL;
 	}
-	nmfk_plan_half_path(j->m, j->n, j->kt, path, &j->planH);
-	nmfk_plan_half_path(j->n, j->m, j->kt, path, &j->planW);
+	nmfk_hplan h, w;
+	nmfk_plan_half_path(j->m, j->n, j->kt, path, &h);
+	nmfk_plan_half_path(j->n, j->m, j->kt, path, &w);
+	if ((size_t)h.S * j->m * j->kt > j->part_cap || (size_t)w.S * j->n * j->kt > j->part_cap) {
+		nmfk_set_error("nmfk_job_set_kernel_path: plan needs more partial space than the job holds "
+			       "(tunables changed after nmfk_job_create)");
+		return NMFK_EINVAL;
+	}
+	j->planH = h;
+	j->planW = w;
 	if (j->chunk_exec) {
 		hipGraphExecDestroy(j->chunk_exec);
 		j->chunk_exec = NULL;
@@ -243,10 +252,26 @@ int nmfk_job_sync(nmfk_job *j)
 static int finish_V(nmfk_job *j)
 {
 	j->colW_valid = 0;
+	j->vt_stale = 0;
 	int e = nmfk_launch_transpose(j->V, j->n, j->m, j->ldv, j->Vt, j->ldvt, j->stream);
 	if (e)
 		return nmfk_hip_fail(e, "transpose V");
 	return nmfk_hip_fail(hipStreamSynchronize(j->stream), "set V");
+}
+
+int nmfk_job_mark_V_changed(nmfk_job *j)
+{
+	j->vt_stale = 1;
+	return 0;
+}
+
+int nmfk_job_refresh_Vt(nmfk_job *j)
+{
+	if (!j->vt_stale)
+		return 0;
+	j->vt_stale = 0;
+	return nmfk_hip_fail(nmfk_launch_transpose(j->V, j->n, j->m, j->ldv, j->Vt, j->ldvt, j->stream),
+			     "transpose V");
 }
 
 int nmfk_job_set_V(nmfk_job *j, const float *V, size_t ldv)
@@ -408,6 +433,8 @@ int nmfk_job_update_H(nmfk_job *j)
 /* update_W(): accum_H = colsum(H) (src/NMF_routines.c:1360), then the W half-step. */
 int nmfk_job_update_W(nmfk_job *j)
 {
+	if (nmfk_job_refresh_Vt(j))
+		return NMFK_EHIP;
 	int e = nmfk_launch_colsum_parts(j->H, j->m, j->kt, j->colH, j->stream);
 	if (!e)
 		e = nmfk_enqueue_half_W(j, NULL, 0);
@@ -483,6 +510,8 @@ int nmfk_job_run(nmfk_job *j, const nmfk_options *opt, nmfk_result *res)
 	}
 	const uint32_t T = opt->niter_test_conv;
 	const uint32_t niter_conv = opt->nIters / T, niter_rem = opt->nIters % T;
+	if (nmfk_job_refresh_Vt(j))
+		return NMFK_EHIP;
 	int e = nmfk_launch_state_reset(j->st, opt->stop_threshold, j->last, j->m, j->stream);
 	if (!e)
 		e = nmfk_launch_colsum_parts(j->W, j->n, j->kt, j->colW, j->stream);
@@ -621,6 +650,58 @@ static int time_loop(nmfk_job *j, uint32_t iters, int kind, double *ms)
 	hipEventDestroy(b);
 	*ms = (double)f / (iters ? iters : 1);
 	return nmfk_hip_fail(e, "timing");
+}
+
+int nmfk_measure_stream(size_t bytes, uint32_t iters, uint32_t blocks, double *read_GBps, double *copy_GBps)
+{
+	float *src = NULL, *dst = NULL;
+	hipEvent_t a = NULL, b = NULL;
+	hipStream_t s = NULL;
+	const size_t nf = bytes / 16 * 4;
+	int e = hipMalloc((void **)&src, nf * 4);
+	if (!e)
+		e = hipMalloc((void **)&dst, nf * 4);
+	if (!e)
+		e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+	if (!e)
+		e = hipEventCreate(&a);
+	if (!e)
+		e = hipEventCreate(&b);
+	if (!e)
+		e = hipMemsetAsync(src, 0, nf * 4, s);
+	if (!e)
+		e = hipMemsetAsync(dst, 0, nf * 4, s);
+	double ms[2] = {0.0, 0.0};
+	for (int kind = 0; kind < 2 && !e; kind++) {
+		e = nmfk_launch_stream(kind, src, dst, nf, blocks, s);   /* warm-up */
+		if (!e)
+			e = hipEventRecord(a, s);
+		for (uint32_t i = 0; i < iters && !e; i++)
+			e = nmfk_launch_stream(kind, src, dst, nf, blocks, s);
+		if (!e)
+			e = hipEventRecord(b, s);
+		if (!e)
+			e = hipEventSynchronize(b);
+		float f = 0.0f;
+		if (!e)
+			e = hipEventElapsedTime(&f, a, b);
+		ms[kind] = (double)f / (iters ? iters : 1);
+	}
+	if (a)
+		hipEventDestroy(a);
+	if (b)
+		hipEventDestroy(b);
+	if (s)
+		hipStreamDestroy(s);
+	hipFree(src);
+	hipFree(dst);
+	if (e)
+		return nmfk_hip_fail(e, "nmfk_measure_stream");
+	if (read_GBps)
+		*read_GBps = (double)(nf * 4) / (ms[0] * 1e-3) / 1e9;
+	if (copy_GBps)
+		*copy_GBps = (double)(2 * nf * 4) / (ms[1] * 1e-3) / 1e9;
+	return 0;
 }
 
 int nmfk_job_time_iterations(nmfk_job *j, uint32_t iters, nmfk_timing *t)

@@ -21,6 +21,7 @@ struct nmfk_job {
 	size_t ldv, ldvt;
 	float *V, *Vt, *W, *H;         /* device */
 	float *part;                   /* partial contractions, S x p x kt */
+	size_t part_cap;               /* floats allocated for part */
 	float *colW, *colH;            /* column-sum partials, one row per NMFK_FIN_ROWS rows */
 	uint32_t ncolW, ncolH;
 	float *denW, *denH;            /* colsum(W), colsum(H): the half-steps' denominators */
@@ -35,7 +36,17 @@ struct nmfk_job {
 	hipGraphExec_t chunk_exec;
 	uint32_t chunk_T;
 	int colW_valid;
+	int vt_stale;                  /* V changed since Vt was built */
 };
+
+/* Rebuilds Vt from V if V was written since (asynchronous on the job's stream). */
+int nmfk_job_refresh_Vt(nmfk_job *j);
+
+/* The factorization the drop-in globals (GPU_setup.h, NMF_routines.h) refer to. */
+extern nmfk_job *nmfk_dropin_job;
+extern uint64_t nmfk_dropin_seed;
+extern uint32_t nmfk_dropin_rng_calls;
+extern int nmfk_dropin_rng_ready;
 
 void nmfk_set_error(const char *fmt, ...);
 int nmfk_hip_fail(int err, const char *what);

@@ -53,6 +53,17 @@ typedef struct nmfk_hplan {
 	uint32_t path;    /* NMFK_PATH_* actually used */
 } nmfk_hplan;
 
+/* Launch-shape tunables of the planner (process-wide; jobs planned afterwards use them). */
+#define NMFK_TUNE_WGS_VALU 0     /* workgroups the VALU partial kernel aims for */
+#define NMFK_TUNE_WGS_MFMA 1     /* workgroups the MFMA partial kernel aims for */
+#define NMFK_TUNE_MIN_STEPS 2    /* minimum row steps per wave before splitting further */
+#define NMFK_TUNE_MAX_SPLITS 3   /* most splits of the contraction range (partials to reduce) */
+#define NMFK_TUNE_MFMA_NA 4      /* MFMA workgroup: waves side by side along the updated factor */
+#define NMFK_TUNE_MFMA_NB 5      /* MFMA workgroup: wave groups splitting the rows of U */
+#define NMFK_TUNE_COUNT 6
+int nmfk_tune_set(int key, int value);
+int nmfk_tune_get(int key);
+
 /* Chooses the partial-kernel grid for updating a p x kt factor against q rows of U. */
 void nmfk_plan_half(uint32_t p, uint32_t q, uint32_t kt, nmfk_hplan *pl);
 void nmfk_plan_half_path(uint32_t p, uint32_t q, uint32_t kt, int path, nmfk_hplan *pl);
@@ -122,8 +133,14 @@ int nmfk_launch_mul_div(float *A, const float *Aux, const float *acc, uint32_t h
 int nmfk_launch_adjust(float *A, uint32_t height, uint32_t width, uint32_t pitch, hipStream_t stream);
 int nmfk_launch_idx_max(const float *A, uint32_t height, uint32_t width, uint32_t pitch, uint32_t *idx,
 			hipStream_t stream);
+/* acc[c] = sum of A[i][c] over <height> rows, c < width; tmp (tmp_items floats, may be
+ * NULL) holds per-split partial rows. */
 int nmfk_launch_colsum_any(const float *A, uint32_t height, uint32_t width, uint32_t pitch,
-			   float *acc, hipStream_t stream);
+			   float *tmp, size_t tmp_items, float *acc, hipStream_t stream);
+
+/* Streaming probe: kind 0 reads nfloats from src (dst receives nothing unless the data
+ * sums to a sentinel), kind 1 copies src to dst; <blocks> workgroups of 256 threads. */
+int nmfk_launch_stream(int kind, const float *src, float *dst, size_t nfloats, unsigned blocks, hipStream_t stream);
 
 #ifdef __cplusplus
 }

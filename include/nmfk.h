@@ -69,6 +69,10 @@ int nmfk_job_gen_V_planted(nmfk_job *job, uint64_t seed, uint32_t kstar, double 
 /* Copies V back to the host (ldv >= m). */
 int nmfk_job_get_V(nmfk_job *job, float *V, size_t ldv);
 
+/* Tells the job that its device V (nmfk_job_info.dV) was written by the caller; the
+ * transposed copy is rebuilt before the next W half-step. */
+int nmfk_job_mark_V_changed(nmfk_job *job);
+
 /* Initial W (stream tag 0) and H (tag 1): uniform (0,1] from Philox4x32-10(seed). */
 int nmfk_job_init_random(nmfk_job *job, uint64_t seed);
 /* Initial W (n x k, ldw) and H (m x k, ldh) from host memory. */
@@ -99,6 +103,16 @@ int nmfk_job_sync(nmfk_job *job);
 int nmfk_job_set_kernel_path(nmfk_job *job, int path);
 int nmfk_job_kernel_path(const nmfk_job *job);
 
+/* Launch-shape tunables of the kernel planner, process-wide; they apply to jobs created
+ * afterwards.  Keys: 0 = workgroups the vector-ALU partial kernel aims for (default 512),
+ * 1 = workgroups the matrix-core partial kernel aims for (default 1024), 2 = minimum row
+ * steps per wave before the contraction range is split further (default 8), 3 = most
+ * splits of the contraction range (default 64), 4 and 5 = matrix-core workgroup shape:
+ * waves side by side along the updated factor x wave groups splitting the contraction
+ * range (default 1 x 4; also 2x2, 4x1, 2x4, 4x2, 1x8). */
+int nmfk_tune_set(int key, int value);
+int nmfk_tune_get(int key);
+
 /* Raw device pointers and shape (for the drop-in layer and for measurements). */
 typedef struct nmfk_job_info {
 	uint32_t n, m, k, kt;
@@ -115,6 +129,11 @@ typedef struct nmfk_timing {
 	double ms_partial_H, ms_finish_H, ms_partial_W, ms_finish_W;
 } nmfk_timing;
 int nmfk_job_time_iterations(nmfk_job *job, uint32_t iters, nmfk_timing *t);
+
+/* Measured streaming ceiling of the current device: a plain float4 read kernel and a
+ * copy kernel over a fresh <bytes> buffer, <iters> timed launches of <blocks> workgroups
+ * each.  Bandwidths count bytes read (read) and read + written (copy). */
+int nmfk_measure_stream(size_t bytes, uint32_t iters, uint32_t blocks, double *read_GBps, double *copy_GBps);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,135 @@
+"""The C-ABI boundary, without a GPU: the library loads, exports every function that
+include/*.h declares, carries no oracle code, and its host-side drop-in functions
+(get_difference, check_arguments, get_padding) behave like the reference's own src/common.c
+built unmodified under oracle/_ref."""
+from __future__ import annotations
+
+import ctypes as C
+import pathlib
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+INCLUDE = ROOT / "include"
+
+
+def declared_functions():
+    names = set()
+    for h in sorted(INCLUDE.glob("*.h")):
+        text = h.read_text()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        text = re.sub(r"#define[^\n]*(\\\n[^\n]*)*", "", text)
+        for m in re.finditer(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(([^;{}]*)\)\s*;", text):
+            name = m.group(1)
+            if name in {"sizeof", "if", "while", "for", "return"}:
+                continue
+            names.add(name)
+    return names
+
+
+def exported_symbols(lib):
+    out = subprocess.run(["nm", "-D", "--defined-only", str(lib)], capture_output=True, text=True, check=True).stdout
+    return {line.split()[-1] for line in out.splitlines() if line.split()[-2:-1] in (["T"], ["W"])}
+
+
+def test_every_declared_function_is_exported(pkg):
+    pkg.load()
+    decl = declared_functions()
+    assert len(decl) > 60, sorted(decl)
+    missing = decl - exported_symbols(pkg.LIB_PATH)
+    assert not missing, sorted(missing)
+
+
+def test_library_has_no_oracle_code(pkg):
+    syms = exported_symbols(pkg.LIB_PATH)
+    assert not any(s.startswith("orc_") for s in syms)
+    ldd = subprocess.run(["ldd", str(pkg.LIB_PATH)], capture_output=True, text=True).stdout
+    assert "oracle" not in ldd and "ref_common" not in ldd
+
+
+def test_headers_compile_as_c(tmp_path):
+    src = tmp_path / "t.c"
+    src.write_text("".join(f'#include "{h.name}"\n' for h in sorted(INCLUDE.glob("*.h"))) + "int main(void){return 0;}\n")
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
+                    f"-I{INCLUDE}", "-c", str(src), "-o", str(tmp_path / "t.o")], check=True)
+
+
+class InputArgs(C.Structure):
+    # struct input_arguments (include/common.h of the reference and of this library)
+    _fields_ = [("filename", C.c_char_p), ("numeric_hdrs", C.c_bool), ("numeric_lbls", C.c_bool),
+                ("input_file_fmt", C.c_int), ("output_file_fmt", C.c_int), ("k", C.c_uint32),
+                ("kp", C.c_uint32), ("nIters", C.c_uint32), ("niter_test_conv", C.c_uint32),
+                ("stop_threshold", C.c_uint32), ("gpu_device", C.c_uint32), ("idx_other_args", C.c_uint32)]
+
+
+def _bind_common(lib):
+    lib.get_difference.restype = C.c_size_t
+    lib.get_difference.argtypes = [np.ctypeslib.ndpointer(np.uint32), np.ctypeslib.ndpointer(np.uint32), C.c_uint32]
+    lib.check_arguments.restype = C.c_int
+    lib.check_arguments.argtypes = [C.c_int, C.POINTER(C.c_char_p), C.POINTER(C.c_bool), C.POINTER(InputArgs)]
+    lib.get_padding.restype = C.c_uint32
+    lib.get_padding.argtypes = [C.c_uint32]
+    lib.set_default_matrix_limits.restype = C.c_int
+    return lib
+
+
+@pytest.fixture(scope="module")
+def libs(pkg, orc):
+    prod = _bind_common(pkg.load())
+    ref = orc.ref_common()
+    if ref is None:
+        pytest.skip("oracle/_ref/libref_common.so not built (reference tree absent)")
+    return prod, _bind_common(ref)
+
+
+def test_get_difference_matches_reference(libs, orc):
+    prod, ref = libs
+    rng = np.random.default_rng(7)
+    for m in (1, 2, 3, 38, 500, 3000):
+        for labels in (2, 9, 64, 300):
+            c = rng.integers(0, labels, m).astype(np.uint32)
+            l = rng.integers(0, labels, m).astype(np.uint32)
+            assert prod.get_difference(c, l, m) == ref.get_difference(c, l, m) == orc.get_difference(c, l)
+
+
+ARGV_CASES = [
+    ["NMF_GPU", "data.txt"],
+    ["NMF_GPU", "data.txt", "-k", "7", "-i", "500", "-j", "5", "-t", "12", "-z", "1"],
+    ["NMF_GPU", "-b", "1", "-e", "0", "-c", "-r", "data.dat", "extra"],
+    ["NMF_GPU", "-K", "33", "-I", "9", "-J", "3", "-T", "4", "f"],
+    ["NMF_GPU", "-k", "1", "f"],        # rank below 2: error
+    ["NMF_GPU", "-i", "0", "f"],        # zero iterations: error
+    ["NMF_GPU", "-j", "x", "f"],        # not a number: error
+    ["NMF_GPU", "-k"],                  # missing argument: error
+    ["NMF_GPU", "-k", "4"],             # no file name: error
+    ["NMF_GPU", "-h", "f"],             # help
+    ["NMF_GPU", "-q", "f"],             # unknown option ignored
+]
+
+
+@pytest.mark.parametrize("argv", ARGV_CASES)
+def test_check_arguments_matches_reference(libs, argv):
+    outs = []
+    for lib in libs:
+        lib.set_default_matrix_limits()
+        arr = (C.c_char_p * (len(argv) + 1))(*[a.encode() for a in argv], None)
+        hv = C.c_bool(False)
+        a = InputArgs()
+        rc = lib.check_arguments(len(argv), arr, C.byref(hv), C.byref(a))
+        fields = None
+        if rc == 0 and not hv.value:
+            fields = tuple((f, getattr(a, f)) for f, _ in InputArgs._fields_)
+        outs.append((rc, hv.value, fields))
+    assert outs[0] == outs[1]
+
+
+def test_get_padding_matches_reference(libs):
+    prod, ref = libs
+    for lib in libs:
+        lib.set_default_matrix_limits()
+    for d in (1, 2, 15, 16, 17, 38, 1000, 4097):
+        assert prod.get_padding(d) == ref.get_padding(d)

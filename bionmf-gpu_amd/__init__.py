@@ -63,6 +63,29 @@ class JobInfo(C.Structure):
                 ("stream", C.c_void_p)]
 
 
+class BatchTuning(C.Structure):
+    """nmfk_batch_tuning (include/nmfk.h)."""
+
+    _fields_ = [("grid_partial", C.c_uint32), ("grid_finish", C.c_uint32), ("order", C.c_uint32),
+                ("chunks_per_poll", C.c_uint32), ("compact_pct", C.c_uint32)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class SweepConfig(C.Structure):
+    _fields_ = [("ks", C.POINTER(C.c_uint32)), ("nk", C.c_uint32), ("restarts", C.c_uint32),
+                ("base_seed", C.c_uint64), ("nIters", C.c_uint32), ("niter_test_conv", C.c_uint32),
+                ("stop_threshold", C.c_uint32), ("mode", C.c_uint32), ("plan_target", C.c_uint32),
+                ("autotune", C.c_uint32), ("autotune_iters", C.c_uint32)]
+
+
+class SweepOut(C.Structure):
+    _fields_ = [("res", C.POINTER(Result)), ("cls", C.POINTER(C.c_uint32)),
+                ("consensus", C.POINTER(C.c_int32)), ("W_H_out", C.POINTER(C.c_float)),
+                ("seconds", C.c_double), ("tune_seconds", C.c_double), ("tuning", BatchTuning)]
+
+
 _F32P = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
 _U32P = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
 
@@ -114,6 +137,22 @@ def load(path: os.PathLike | str | None = None):
     _sig(lib, "nmfk_tune_set", i32, [i32, i32])
     _sig(lib, "nmfk_tune_get", i32, [i32])
     _sig(lib, "nmfk_measure_stream", i32, [sz, u32, u32, C.POINTER(dbl), C.POINTER(dbl)])
+    _sig(lib, "nmfk_job_create_shared", vp, [vp, u32])
+    _sig(lib, "nmfk_job_replan", i32, [vp, u32])
+    _sig(lib, "nmfk_job_plan_get", i32, [vp, _U32P])
+    _sig(lib, "nmfk_batch_default_tuning", None, [C.POINTER(BatchTuning)])
+    _sig(lib, "nmfk_batch_create", vp, [u32, u32])
+    _sig(lib, "nmfk_batch_destroy", None, [vp])
+    _sig(lib, "nmfk_batch_set_tuning", i32, [vp, C.POINTER(BatchTuning)])
+    _sig(lib, "nmfk_batch_get_tuning", i32, [vp, C.POINTER(BatchTuning)])
+    _sig(lib, "nmfk_batch_add", i32, [vp, vp, C.POINTER(Options)])
+    _sig(lib, "nmfk_batch_active", u32, [vp])
+    _sig(lib, "nmfk_batch_step", i32, [vp, u32])
+    _sig(lib, "nmfk_batch_poll", i32, [vp, C.POINTER(vp), C.POINTER(Result), u32, C.POINTER(u32)])
+    _sig(lib, "nmfk_batch_autotune", i32, [vp, u32, C.POINTER(BatchTuning), C.POINTER(dbl)])
+    _sig(lib, "nmfk_sweep_default_config", None, [C.POINTER(SweepConfig)])
+    _sig(lib, "nmfk_sweep_seed", u64, [u64, u32, u32])
+    _sig(lib, "nmfk_sweep_run", i32, [vp, C.POINTER(SweepConfig), C.POINTER(SweepOut)])
     _lib = lib
     return lib
 
@@ -166,14 +205,33 @@ def default_options(k: int = 2, nIters: int = 2000, niter_test_conv: int = 10,
 
 
 class Job:
-    """One factorization V (n x m) ~ W (n x k) . H^T (H is m x k), resident on the GPU."""
+    """One factorization V (n x m) ~ W (n x k) . H^T (H is m x k), resident on the GPU.
 
-    def __init__(self, n: int, m: int, k: int):
+    Job(n, m, k) owns its V; Job.shared(src, k) reads the V of <src> without a copy."""
+
+    def __init__(self, n: int, m: int, k: int, _src: "Job | None" = None):
         self.lib = load()
         self.n, self.m, self.k = int(n), int(m), int(k)
-        self.h = self.lib.nmfk_job_create(self.n, self.m, self.k)
+        self._src = _src
+        if _src is None:
+            self.h = self.lib.nmfk_job_create(self.n, self.m, self.k)
+        else:
+            self.h = self.lib.nmfk_job_create_shared(_src.h, self.k)
         if not self.h:
             raise NmfkError(f"nmfk_job_create failed: {last_error()}")
+
+    @classmethod
+    def shared(cls, src: "Job", k: int) -> "Job":
+        return cls(src.n, src.m, k, _src=src)
+
+    def replan(self, target_wgs: int):
+        _check(self.lib.nmfk_job_replan(self.h, target_wgs), "replan")
+
+    def plan(self) -> dict:
+        out = np.zeros(8, np.uint32)
+        self.lib.nmfk_job_plan_get(self.h, out)
+        keys = ("nab", "S", "bchunk", "path")
+        return {"H": dict(zip(keys, map(int, out[:4]))), "W": dict(zip(keys, map(int, out[4:])))}
 
     def close(self):
         if self.h:
@@ -275,3 +333,124 @@ class Job:
         t = Timing()
         _check(self.lib.nmfk_job_time_iterations(self.h, iters, C.byref(t)), "time_iterations")
         return {f: getattr(t, f) for f, _ in Timing._fields_}
+
+
+class Batch:
+    """nmfk_batch: continuous batching of many jobs (include/nmfk.h)."""
+
+    def __init__(self, niter_test_conv: int = 10, capacity: int = 256):
+        self.lib = load()
+        self.T = niter_test_conv
+        self.cap = capacity
+        self.h = self.lib.nmfk_batch_create(niter_test_conv, capacity)
+        if not self.h:
+            raise NmfkError(f"nmfk_batch_create failed: {last_error()}")
+        self._jobs = {}
+
+    def close(self):
+        if self.h:
+            self.lib.nmfk_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add(self, job: Job, nIters=2000, stop_threshold=40) -> int:
+        opt = Options(job.k, nIters, self.T, stop_threshold)
+        slot = self.lib.nmfk_batch_add(self.h, job.h, C.byref(opt))
+        if slot < 0:
+            raise NmfkError(f"batch add failed: {last_error()}")
+        self._jobs[job.h] = job
+        return slot
+
+    @property
+    def active(self) -> int:
+        return self.lib.nmfk_batch_active(self.h)
+
+    def step(self, chunks: int = 1):
+        _check(self.lib.nmfk_batch_step(self.h, chunks), "batch step")
+
+    def poll(self):
+        """[(job, result dict)] of the jobs that finished."""
+        done = (C.c_void_p * self.cap)()
+        res = (Result * self.cap)()
+        nd = C.c_uint32(0)
+        _check(self.lib.nmfk_batch_poll(self.h, done, res, self.cap, C.byref(nd)), "batch poll")
+        return [(self._jobs.pop(done[i]), res[i].as_dict()) for i in range(nd.value)]
+
+    def tuning(self) -> dict:
+        t = BatchTuning()
+        self.lib.nmfk_batch_get_tuning(self.h, C.byref(t))
+        return t.as_dict()
+
+    def set_tuning(self, **kw):
+        t = BatchTuning()
+        self.lib.nmfk_batch_get_tuning(self.h, C.byref(t))
+        for k_, v in kw.items():
+            setattr(t, k_, v)
+        _check(self.lib.nmfk_batch_set_tuning(self.h, C.byref(t)), "batch tuning")
+
+    def autotune(self, iters: int = 20):
+        t = BatchTuning()
+        ms = C.c_double(0.0)
+        _check(self.lib.nmfk_batch_autotune(self.h, iters, C.byref(t), C.byref(ms)), "autotune")
+        return t.as_dict(), ms.value
+
+    def run(self, chunks_per_poll: int = 1):
+        """Steps until every job has finished; returns [(job, result)] in finishing order."""
+        out = []
+        while self.active:
+            self.step(chunks_per_poll)
+            out += self.poll()
+        return out
+
+
+SWEEP_BATCHED, SWEEP_ONE_AT_A_TIME = 0, 1
+
+
+def sweep_seed(base: int, k: int, restart: int) -> int:
+    return load().nmfk_sweep_seed(base, k, restart)
+
+
+def sweep(src: Job, ks, restarts: int, base_seed: int, nIters=2000, niter_test_conv=10,
+          stop_threshold=40, mode=SWEEP_BATCHED, plan_target=0, autotune=False, autotune_iters=20,
+          want_factors=False) -> dict:
+    """nmfk_sweep_run: every (k, restart) job on src's V; returns results, final class vectors
+    (nk x restarts x m), consensus matrices (nk x m x m int32) and timings."""
+    lib = load()
+    ks = np.ascontiguousarray(ks, dtype=np.uint32)
+    nk, m = len(ks), src.m
+    cfg = SweepConfig()
+    lib.nmfk_sweep_default_config(C.byref(cfg))
+    cfg.ks = ks.ctypes.data_as(C.POINTER(C.c_uint32))
+    cfg.nk, cfg.restarts, cfg.base_seed = nk, restarts, base_seed
+    cfg.nIters, cfg.niter_test_conv, cfg.stop_threshold = nIters, niter_test_conv, stop_threshold
+    cfg.mode, cfg.plan_target = mode, plan_target
+    cfg.autotune, cfg.autotune_iters = int(bool(autotune)), autotune_iters
+    nj = nk * restarts
+    res = (Result * nj)()
+    cls = np.zeros((nk, restarts, m), np.uint32)
+    cons = np.zeros((nk, m, m), np.int32)
+    out = SweepOut()
+    out.res = C.cast(res, C.POINTER(Result))
+    out.cls = cls.ctypes.data_as(C.POINTER(C.c_uint32))
+    out.consensus = cons.ctypes.data_as(C.POINTER(C.c_int32))
+    fac = None
+    if want_factors:
+        fac = np.zeros(int(sum((src.n + m) * int(k) for k in ks) * restarts), np.float32)
+        out.W_H_out = fac.ctypes.data_as(C.POINTER(C.c_float))
+    _check(lib.nmfk_sweep_run(src.h, C.byref(cfg), C.byref(out)), "sweep")
+    r = {"results": [res[i].as_dict() for i in range(nj)], "cls": cls, "consensus": cons,
+         "seconds": out.seconds, "tune_seconds": out.tune_seconds, "tuning": out.tuning.as_dict()}
+    if want_factors:
+        facs, o = [], 0
+        for k in ks:
+            for _ in range(restarts):
+                W = fac[o:o + src.n * int(k)].reshape(src.n, int(k)); o += src.n * int(k)
+                H = fac[o:o + m * int(k)].reshape(m, int(k)); o += m * int(k)
+                facs.append((W, H))
+        r["factors"] = facs
+    return r

@@ -189,8 +189,10 @@ int check_arguments(int argc, char const *const *argv, bool *help_requested, str
 	a.stop_threshold = DEFAULT_STOP_THRESHOLD;
 	a.gpu_device = DEFAULT_GPU_DEVICE;
 
+	/* optind = 0 makes glibc's getopt re-initialise completely (argument permutation state
+	 * included), so a previous parse in the process cannot leak into this one. */
 	optarg = NULL;
-	optind = 1;
+	optind = 0;
 	opterr = 0;
 	optopt = 0;
 	int opt;

@@ -120,7 +120,25 @@ static int dalloc(void **p, size_t bytes, const char *what)
 	return nmfk_hip_fail(hipMalloc(p, bytes), what);
 }
 
+static nmfk_job *job_create(uint32_t n, uint32_t m, uint32_t k, nmfk_job *src);
+
 nmfk_job *nmfk_job_create(uint32_t n, uint32_t m, uint32_t k)
+{
+	return job_create(n, m, k, NULL);
+}
+
+nmfk_job *nmfk_job_create_shared(nmfk_job *src, uint32_t k)
+{
+	if (!src) {
+		nmfk_set_error("nmfk_job_create_shared: no source job");
+		return NULL;
+	}
+	if (nmfk_job_refresh_Vt(src) || nmfk_job_sync(src))
+		return NULL;
+	return job_create(src->n, src->m, k, src);
+}
+
+static nmfk_job *job_create(uint32_t n, uint32_t m, uint32_t k, nmfk_job *src)
 {
 	if (n < 1 || m < 1 || k < 1 || k > NMFK_KMAX) {
 		nmfk_set_error("nmfk_job_create: invalid shape n=%u m=%u k=%u (1 <= k <= %d)", n, m, k, NMFK_KMAX);
@@ -146,8 +164,14 @@ nmfk_job *nmfk_job_create(uint32_t n, uint32_t m, uint32_t k)
 	j->ncolH = (m + NMFK_FIN_ROWS - 1) / NMFK_FIN_ROWS;
 	j->ndot = nmfk_dot_kl_blocks(n, m);
 	int e = 0;
-	e |= dalloc((void **)&j->V, 4 * L.n_V, "hipMalloc(V)");
-	if (!e) e |= dalloc((void **)&j->Vt, 4 * L.n_Vt, "hipMalloc(Vt)");
+	j->owns_V = src == NULL;
+	if (src) {
+		j->V = src->V;
+		j->Vt = src->Vt;
+	} else {
+		e |= dalloc((void **)&j->V, 4 * L.n_V, "hipMalloc(V)");
+		if (!e) e |= dalloc((void **)&j->Vt, 4 * L.n_Vt, "hipMalloc(Vt)");
+	}
 	if (!e) e |= dalloc((void **)&j->W, 4 * L.n_W, "hipMalloc(W)");
 	if (!e) e |= dalloc((void **)&j->H, 4 * L.n_H, "hipMalloc(H)");
 	if (!e) e |= dalloc((void **)&j->part, 4 * L.n_part, "hipMalloc(partials)");
@@ -174,8 +198,10 @@ nmfk_job *nmfk_job_create(uint32_t n, uint32_t m, uint32_t k)
 	/* Zero W and H so their padding columns start (and stay) at zero. */
 	hipMemsetAsync(j->W, 0, 4 * L.n_W, j->stream);
 	hipMemsetAsync(j->H, 0, 4 * L.n_H, j->stream);
-	hipMemsetAsync(j->V, 0, 4 * L.n_V, j->stream);
-	hipMemsetAsync(j->Vt, 0, 4 * L.n_Vt, j->stream);
+	if (j->owns_V) {
+		hipMemsetAsync(j->V, 0, 4 * L.n_V, j->stream);
+		hipMemsetAsync(j->Vt, 0, 4 * L.n_Vt, j->stream);
+	}
 	if (nmfk_hip_fail(hipStreamSynchronize(j->stream), "job create")) {
 		nmfk_job_destroy(j);
 		return NULL;
@@ -191,7 +217,9 @@ void nmfk_job_destroy(nmfk_job *j)
 		hipStreamSynchronize(j->stream);
 	if (j->chunk_exec)
 		hipGraphExecDestroy(j->chunk_exec);
-	hipFree(j->V), hipFree(j->Vt), hipFree(j->W), hipFree(j->H), hipFree(j->part);
+	if (j->owns_V)
+		hipFree(j->V), hipFree(j->Vt);
+	hipFree(j->W), hipFree(j->H), hipFree(j->part);
 	hipFree(j->colW), hipFree(j->colH), hipFree(j->denW), hipFree(j->dot), hipFree(j->cls), hipFree(j->last);
 	hipFree(j->st);
 	if (j->h_st)
@@ -237,6 +265,31 @@ int nmfk_job_set_kernel_path(nmfk_job *j, int path)
 	return 0;
 }
 
+int nmfk_job_replan(nmfk_job *j, uint32_t target_wgs)
+{
+	nmfk_hplan h, w;
+	nmfk_plan_half_target(j->m, j->n, j->kt, (int)j->planH.path, target_wgs, &h);
+	nmfk_plan_half_target(j->n, j->m, j->kt, (int)j->planW.path, target_wgs, &w);
+	if ((size_t)h.S * j->m * j->kt > j->part_cap || (size_t)w.S * j->n * j->kt > j->part_cap) {
+		nmfk_set_error("nmfk_job_replan: plan needs more partial space than the job holds");
+		return NMFK_EINVAL;
+	}
+	j->planH = h;
+	j->planW = w;
+	if (j->chunk_exec) {
+		hipGraphExecDestroy(j->chunk_exec);
+		j->chunk_exec = NULL;
+	}
+	return 0;
+}
+
+int nmfk_job_plan_get(const nmfk_job *j, uint32_t out[8])
+{
+	out[0] = j->planH.nab, out[1] = j->planH.S, out[2] = j->planH.bchunk, out[3] = j->planH.path;
+	out[4] = j->planW.nab, out[5] = j->planW.S, out[6] = j->planW.bchunk, out[7] = j->planW.path;
+	return 0;
+}
+
 int nmfk_job_kernel_path(const nmfk_job *j)
 {
 	return (int)j->planW.path;
@@ -276,6 +329,10 @@ int nmfk_job_refresh_Vt(nmfk_job *j)
 
 int nmfk_job_set_V(nmfk_job *j, const float *V, size_t ldv)
 {
+	if (!j->owns_V) {
+		nmfk_set_error("V of a job created by nmfk_job_create_shared belongs to its source job");
+		return NMFK_EINVAL;
+	}
 	if (ldv < j->m) {
 		nmfk_set_error("nmfk_job_set_V: ldv=%zu < m=%u", ldv, j->m);
 		return NMFK_EINVAL;
@@ -289,6 +346,10 @@ int nmfk_job_set_V(nmfk_job *j, const float *V, size_t ldv)
 
 int nmfk_job_set_V_device(nmfk_job *j, const float *dV, size_t ldv)
 {
+	if (!j->owns_V) {
+		nmfk_set_error("V of a job created by nmfk_job_create_shared belongs to its source job");
+		return NMFK_EINVAL;
+	}
 	if (ldv < j->m) {
 		nmfk_set_error("nmfk_job_set_V_device: ldv=%zu < m=%u", ldv, j->m);
 		return NMFK_EINVAL;
@@ -302,6 +363,10 @@ int nmfk_job_set_V_device(nmfk_job *j, const float *dV, size_t ldv)
 
 int nmfk_job_gen_V_lognormal(nmfk_job *j, uint64_t seed, double mu, double sigma)
 {
+	if (!j->owns_V) {
+		nmfk_set_error("V of a job created by nmfk_job_create_shared belongs to its source job");
+		return NMFK_EINVAL;
+	}
 	int e = nmfk_launch_gen_lognormal(j->V, j->n, j->m, (uint32_t)j->ldv, seed, mu, sigma, j->stream);
 	if (e)
 		return nmfk_hip_fail(e, "gen lognormal");
@@ -310,6 +375,10 @@ int nmfk_job_gen_V_lognormal(nmfk_job *j, uint64_t seed, double mu, double sigma
 
 int nmfk_job_gen_V_planted(nmfk_job *j, uint64_t seed, uint32_t kstar, double sigma)
 {
+	if (!j->owns_V) {
+		nmfk_set_error("V of a job created by nmfk_job_create_shared belongs to its source job");
+		return NMFK_EINVAL;
+	}
 	double *scratch = NULL;
 	int e = dalloc((void **)&scratch, 8 * (size_t)kstar * ((size_t)j->n + j->m), "hipMalloc(planted)");
 	if (e)
@@ -512,7 +581,7 @@ int nmfk_job_run(nmfk_job *j, const nmfk_options *opt, nmfk_result *res)
 	const uint32_t niter_conv = opt->nIters / T, niter_rem = opt->nIters % T;
 	if (nmfk_job_refresh_Vt(j))
 		return NMFK_EHIP;
-	int e = nmfk_launch_state_reset(j->st, opt->stop_threshold, j->last, j->m, j->stream);
+	int e = nmfk_launch_state_reset(j->st, opt->stop_threshold, UINT32_MAX, j->last, j->m, j->stream);
 	if (!e)
 		e = nmfk_launch_colsum_parts(j->W, j->n, j->kt, j->colW, j->stream);
 	if (e)
@@ -561,7 +630,7 @@ int nmfk_job_run(nmfk_job *j, const nmfk_options *opt, nmfk_result *res)
 	const nmfk_state s = j->h_st[0];
 	j->colW_valid = 1;
 	if (res) {
-		res->converged = s.converged;
+		res->converged = s.converged == 1;
 		res->ntests = s.ntests;
 		res->last_diff = s.last_diff;
 		res->iterations = s.converged ? s.stop_test * T : opt->nIters;

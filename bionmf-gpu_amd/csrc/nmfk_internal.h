@@ -37,6 +37,7 @@ struct nmfk_job {
 	uint32_t chunk_T;
 	int colW_valid;
 	int vt_stale;                  /* V changed since Vt was built */
+	int owns_V;                    /* 0: V and Vt belong to another job (nmfk_job_create_shared) */
 };
 
 /* Rebuilds Vt from V if V was written since (asynchronous on the job's stream). */

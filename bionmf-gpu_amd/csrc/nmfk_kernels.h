@@ -21,12 +21,13 @@ extern "C" {
 /* Device-side convergence state of one factorization (src/NMF_GPU.c:522-711). */
 typedef struct nmfk_state {
 	uint32_t inc;         /* unchanged tests in a row */
-	uint32_t converged;   /* set by the test that meets the stop rule; every later launch is a no-op */
+	uint32_t converged;   /* 1: the stop rule was met; 2: test budget spent without converging;
+	                         nonzero makes every later launch of the job a no-op */
 	uint32_t ntests;      /* tests performed */
 	uint32_t stop_test;   /* 1-based test index that converged (0 = none) */
 	uint64_t last_diff;   /* connectivity difference of the most recent test */
 	uint32_t stop_threshold;
-	uint32_t pad_;
+	uint32_t max_tests;   /* tests after which an unconverged job is done (UINT32_MAX: no limit) */
 } nmfk_state;
 
 /* Largest rank handled by the kernels. */
@@ -67,6 +68,8 @@ int nmfk_tune_get(int key);
 /* Chooses the partial-kernel grid for updating a p x kt factor against q rows of U. */
 void nmfk_plan_half(uint32_t p, uint32_t q, uint32_t kt, nmfk_hplan *pl);
 void nmfk_plan_half_path(uint32_t p, uint32_t q, uint32_t kt, int path, nmfk_hplan *pl);
+/* Same, aiming for <target> workgroups instead of the process tunable (0 = tunable). */
+void nmfk_plan_half_target(uint32_t p, uint32_t q, uint32_t kt, int path, uint32_t target, nmfk_hplan *pl);
 
 /*
  * Half-step, first kernel: part[s][a][c] = sum over b in split s of
@@ -104,8 +107,53 @@ int nmfk_launch_test(float *H, uint32_t m, uint32_t k, uint32_t kt, uint32_t *cl
 		     nmfk_state *st, hipStream_t stream);
 
 /* Resets a state block and zeroes last[] (the reference's initial all-zeros vector). */
-int nmfk_launch_state_reset(nmfk_state *st, uint32_t stop_threshold, uint32_t *last, uint32_t m,
-			    hipStream_t stream);
+int nmfk_launch_state_reset(nmfk_state *st, uint32_t stop_threshold, uint32_t max_tests, uint32_t *last,
+			    uint32_t m, hipStream_t stream);
+
+/*
+ * Batched launches: one grid advances many jobs.  Each job is described on the device by an
+ * nmfk_bjob; a launch walks a table of tiles (job, bx, by), each tile being exactly the
+ * workgroup (bx, by) of that job's own single-job grid, so a job computes the same bits
+ * alone and in a batch.  Workgroups loop over tiles t = blockIdx.x, + gridDim.x, ...
+ * (tiles per workgroup = ceil(ntiles / grid)); tiles of a job whose state is nonzero
+ * return at once.
+ */
+typedef struct nmfk_bjob {
+	const float *V, *Vt;
+	float *W, *H, *part;
+	float *colW, *colH, *denW, *denH;
+	nmfk_state *st;
+	uint32_t *cls, *last;
+	uint64_t ldv, ldvt;
+	uint32_t n, m, k, kt;
+	uint32_t ncolW, ncolH;
+	nmfk_hplan planH, planW;
+	uint32_t live;        /* 0: empty slot, every tile of it returns without reading the job */
+	uint32_t pad_;
+} nmfk_bjob;
+
+typedef struct nmfk_tile {
+	uint32_t job, bx, by, pad_;
+} nmfk_tile;
+
+/* Largest padded rank a batched launch handles (its LDS is sized for it). */
+#define NMFK_BATCH_KTMAX 32
+
+/* half: 0 = H half (U = V, A = H, B = W), 1 = W half (U = Vt, A = W, B = H). */
+int nmfk_launch_batch_partial(const nmfk_bjob *jobs, const nmfk_tile *tiles, uint32_t ntiles, int half,
+			      uint32_t grid, hipStream_t stream);
+int nmfk_launch_batch_finish(const nmfk_bjob *jobs, const nmfk_tile *tiles, uint32_t ntiles, int half,
+			     int clamp, uint32_t grid, hipStream_t stream);
+/* Test: rows tiles (job, row block of NMFK_TEST_ROWS), then one final workgroup per job. */
+#define NMFK_TEST_ROWS 32
+int nmfk_launch_batch_test(const nmfk_bjob *jobs, const nmfk_tile *row_tiles, uint32_t nrow_tiles,
+			   uint32_t njobs, uint32_t grid, hipStream_t stream);
+
+/* out[t] = *jobs[t].st (zeros for empty slots), t < njobs. */
+int nmfk_launch_batch_gather_states(const nmfk_bjob *jobs, uint32_t njobs, nmfk_state *out, hipStream_t stream);
+
+/* C[i][j] += (cls[i] == cls[j]) for an m x m int32 consensus matrix. */
+int nmfk_launch_consensus_add(const uint32_t *cls, uint32_t m, int32_t *C, hipStream_t stream);
 
 /* A[i][j] = uniform (0,1] from Philox4x32-10 (seed, tag, i*width + j); padding columns = 0. */
 int nmfk_launch_uniform(float *A, uint32_t height, uint32_t width, uint32_t pitch,

@@ -110,10 +110,16 @@ __device__ __forceinline__ double wave_sum_d(double v)
 // Denominator of a half-step: den[c] = sum over the nparts column-sum partials of the
 // other factor.  Four interleaved ascending partial sums per column, combined in a fixed
 // order, so every kernel that forms it gets the same bits.  Needs 4*KT <= blockDim.x.
+// Workgroup size of every half-step kernel (single-job and batched launches alike).
+#define NMFK_WG 256
+
+// den[c] = sum_g parts[g][c]: four interleaved running sums per column, added in a fixed
+// order at the end.  s4 is 4*KT floats of LDS; needs 4*KT <= NMFK_WG threads.
 template <int KT>
-__device__ __forceinline__ void den_reduce(const float *__restrict__ parts, int nparts, float *__restrict__ den)
+__device__ __forceinline__ void den_reduce(const float *__restrict__ parts, int nparts, float *__restrict__ den,
+					   float *__restrict__ s4)
 {
-	__shared__ float s4[4 * KT];
+	static_assert(4 * KT <= NMFK_WG, "den_reduce: one thread per (column, lane of 4)");
 	const int t = threadIdx.x;
 	if (t < 4 * KT) {
 		const int c = t >> 2, j = t & 3;
@@ -138,33 +144,35 @@ __device__ __forceinline__ void den_reduce(const float *__restrict__ parts, int 
 		den[t] = ((s4[4 * t] + s4[4 * t + 1]) + s4[4 * t + 2]) + s4[4 * t + 3];
 }
 
-// One finish block = NMFK_FIN_ROWS (64) rows of A with fin_tpr(KT) threads per row, each
-// thread fin_q(KT) consecutive columns.  Column sums of the block's rows (ascending rows)
-// are written as one partial row; shared by k_finish and k_colsum_parts so both give the
-// same bits.
-__host__ __device__ constexpr int fin_tpr(int kt)
+// One finish block = NMFK_FIN_ROWS (64) rows of A with 4 threads per row, each thread
+// KT/4 consecutive columns (NMFK_WG threads).  Column sums of the block's rows (ascending
+// rows) are written as one partial row; shared by the finish and column-sum kernels so
+// both give the same bits.
+__host__ __device__ constexpr int fin_tpr(int)
 {
-	return kt >= 16 ? 16 : kt;
+	return 4;
 }
 __host__ __device__ constexpr int fin_q(int kt)
 {
 	return kt / fin_tpr(kt);
 }
+static_assert(NMFK_FIN_ROWS * 4 == NMFK_WG, "finish block shape");
 
+// tile: NMFK_FIN_ROWS x (KT + 1) floats of LDS.
 template <int KT>
-__device__ __forceinline__ void block_colsum_store(const float (&x)[fin_q(KT)], float *__restrict__ out)
+__device__ __forceinline__ void block_colsum_store(const float (&x)[fin_q(KT)], float *__restrict__ out,
+						   float *__restrict__ tile)
 {
 	constexpr int TPR = fin_tpr(KT), Q = fin_q(KT);
-	__shared__ float tile[NMFK_FIN_ROWS][KT + 1];
 	const int row = threadIdx.x / TPR, q = threadIdx.x % TPR;
 #pragma unroll
 	for (int i = 0; i < Q; i++)
-		tile[row][q * Q + i] = x[i];
+		tile[row * (KT + 1) + q * Q + i] = x[i];
 	__syncthreads();
 	if (threadIdx.x < KT) {
 		float s = 0.0f;
 		for (int r = 0; r < NMFK_FIN_ROWS; r++)
-			s += tile[r][threadIdx.x];
+			s += tile[r * (KT + 1) + threadIdx.x];
 		out[threadIdx.x] = s;
 	}
 }
@@ -193,29 +201,51 @@ __device__ __forceinline__ void load_u(const float *__restrict__ src, int avail,
 		u[r] = (r < avail) ? u[r] : 0.0f;
 }
 
-// part[blockIdx.y][a][c] = sum_{b in split} B[b][c] * U[b][a] / (A[a] . B[b])
-template <int KT, int AP, int NW>
-__global__ __launch_bounds__(64 * NW) void k_partial(const float *__restrict__ U, size_t ldu,
-						     const float *__restrict__ A,
-						     const float *__restrict__ B, int p, int q,
-						     int bchunk, float *__restrict__ part,
-						     const float *__restrict__ colB, int ncolB,
-						     float *__restrict__ den,
-						     const nmfk_state *__restrict__ st)
+// Arguments of one half-step's partial kernel: update A (p x KT) against B (q x KT)
+// through U (q x p, leading dimension ldu).
+struct half_args {
+	const float *U;
+	size_t ldu;
+	const float *A, *B;
+	int p, q, bchunk, nab;
+	float *part;
+	const float *colB;
+	int ncolB;
+	float *den;
+};
+
+// LDS floats the partial bodies need (VALU: NW/2 wave slots of AP*KT*64; MFMA: NB-1 wave
+// groups of NA*4*KT/16*4*64).
+__host__ __device__ constexpr int smem_valu(int kt, int ap, int nw)
 {
-	if (st != nullptr && st->converged)
-		return;
-	// One extra column of workgroups (x == gridDim.x - 1): its first workgroup reduces the
-	// other factor's column-sum partials into the denominators the finish kernel divides
-	// by, off the critical path of the streaming workgroups; the rest exit.
-	if (blockIdx.x == gridDim.x - 1) {
-		if (blockIdx.y == 0)
-			den_reduce<KT>(colB, ncolB, den);
+	return (nw / 2 > 0 ? nw / 2 : 1) * ap * kt * 64;
+}
+__host__ __device__ constexpr int smem_mfma(int kt, int na, int nb)
+{
+	return (nb > 1 ? nb - 1 : 1) * na * 4 * (kt / 16) * 4 * 64;
+}
+
+// Workgroup (bx, by) of a half-step's partial grid, bx in [0, nab], by in [0, S):
+// part[by][a][c] = sum_{b in split by} B[b][c] * U[b][a] / (A[a] . B[b]).
+// Column bx == nab is the denominator column: its first workgroup reduces the other
+// factor's column-sum partials into the denominators the finish kernel divides by, off
+// the critical path of the streaming workgroups; the rest return at once.
+template <int KT, int AP, int NW>
+__device__ __forceinline__ void partial_valu_body(int bx, int by, const half_args &h, float *__restrict__ smem)
+{
+	const float *__restrict__ U = h.U;
+	const size_t ldu = h.ldu;
+	const float *__restrict__ A = h.A;
+	const float *__restrict__ B = h.B;
+	const int p = h.p, q = h.q, bchunk = h.bchunk;
+	if (bx == h.nab) {
+		if (by == 0)
+			den_reduce<KT>(h.colB, h.ncolB, h.den, smem);
 		return;
 	}
 	const int lane = threadIdx.x & 63;
 	const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-	const int a0 = (blockIdx.x * 64 + lane) * AP;
+	const int a0 = (bx * 64 + lane) * AP;
 	const int avail = p - a0;
 
 	float av[AP][KT], acc[AP][KT];
@@ -230,7 +260,7 @@ __global__ __launch_bounds__(64 * NW) void k_partial(const float *__restrict__ U
 		}
 	}
 
-	const int b_begin = blockIdx.y * bchunk;
+	const int b_begin = by * bchunk;
 	const int b_end = min(q, b_begin + bchunk);
 	const float *Ucol = U + (avail > 0 ? a0 : 0);
 	// One row of U is prefetched ahead so its latency overlaps the current row's math.
@@ -271,7 +301,7 @@ __global__ __launch_bounds__(64 * NW) void k_partial(const float *__restrict__ U
 
 	// Fixed-order tree over the NW waves of the workgroup.
 	constexpr int E = AP * KT;
-	__shared__ float red[(NW / 2 > 0 ? NW / 2 : 1) * E * 64];
+	float *__restrict__ red = smem;
 #pragma unroll
 	for (int half = NW / 2; half >= 1; half >>= 1) {
 		if (wave >= half && wave < 2 * half) {
@@ -294,7 +324,7 @@ __global__ __launch_bounds__(64 * NW) void k_partial(const float *__restrict__ U
 		__syncthreads();
 	}
 	if (wave == 0) {
-		float *dst = part + ((size_t)blockIdx.y * p + a0) * KT;
+		float *dst = h.part + ((size_t)by * p + a0) * KT;
 #pragma unroll
 		for (int r = 0; r < AP; r++)
 			if (r < avail) {
@@ -305,6 +335,15 @@ __global__ __launch_bounds__(64 * NW) void k_partial(const float *__restrict__ U
 							make_float4(acc[r][c], acc[r][c + 1], acc[r][c + 2], acc[r][c + 3]);
 			}
 	}
+}
+
+template <int KT, int AP, int NW>
+__global__ __launch_bounds__(64 * NW) void k_partial(half_args h, const nmfk_state *__restrict__ st)
+{
+	__shared__ float smem[smem_valu(KT, AP, NW)];
+	if (st != nullptr && st->converged)
+		return;
+	partial_valu_body<KT, AP, NW>(blockIdx.x, blockIdx.y, h, smem);
 }
 
 // --------------------------------------------------------------------------------------
@@ -354,22 +393,19 @@ __device__ __forceinline__ float bload1(__amdgpu_buffer_rsrc_t r, int voff, int 
 // NB groups of them taking the split's 16-row blocks round-robin; the NB partial sums of
 // each row a are added through LDS in a fixed order.
 template <int KT, int NA, int NB>
-__global__ __launch_bounds__(64 * NA * NB) void k_partial_mfma(const float *__restrict__ U, size_t ldu,
-								const float *__restrict__ A,
-								const float *__restrict__ B, int p, int q,
-								int bchunk, float *__restrict__ part,
-								const float *__restrict__ colB, int ncolB,
-								float *__restrict__ den,
-								const nmfk_state *__restrict__ st)
+__device__ __forceinline__ void partial_mfma_body(int bx, int by, const half_args &ha, float *__restrict__ smem)
 {
 	static_assert(KT % 16 == 0, "MFMA path needs KT % 16 == 0");
 	constexpr int H = KT / 16;     // 16-column groups of the rank
-	if (st != nullptr && st->converged)
-		return;
-	// Extra workgroup column: the denominators (see k_partial).
-	if (blockIdx.x == gridDim.x - 1) {
-		if (blockIdx.y == 0)
-			den_reduce<KT>(colB, ncolB, den);
+	const float *__restrict__ U = ha.U;
+	const size_t ldu = ha.ldu;
+	const float *__restrict__ A = ha.A;
+	const float *__restrict__ B = ha.B;
+	const int p = ha.p, q = ha.q, bchunk = ha.bchunk;
+	// Denominator column (see partial_valu_body).
+	if (bx == ha.nab) {
+		if (by == 0)
+			den_reduce<KT>(ha.colB, ha.ncolB, ha.den, smem);
 		return;
 	}
 
@@ -377,7 +413,7 @@ __global__ __launch_bounds__(64 * NA * NB) void k_partial_mfma(const float *__re
 	const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
 	const int wa = wave % NA, wb = wave / NA;
 	const int j = lane & 15, g = lane >> 4;
-	const int a0 = (blockIdx.x * NA + wa) * 64;
+	const int a0 = (bx * NA + wa) * 64;
 	const bool idle = a0 >= p;               // wave-uniform; idle waves still join the LDS step
 	const int aj = a0 + 4 * j;               // first of this lane's 4 rows a (one per tile)
 	const int avail = p - aj;                // tiles t < avail are real rows
@@ -401,7 +437,7 @@ __global__ __launch_bounds__(64 * NA * NB) void k_partial_mfma(const float *__re
 		for (int h = 0; h < H; h++)
 			acc[t][h] = floatx4{0.f, 0.f, 0.f, 0.f};
 
-	const int b_begin = blockIdx.y * bchunk;
+	const int b_begin = by * bchunk;
 	const int b_end = min(q, b_begin + bchunk);
 
 	// Lanes past p read column 0 (in bounds); their rows a only reach MFMA output rows of
@@ -567,7 +603,7 @@ __global__ __launch_bounds__(64 * NA * NB) void k_partial_mfma(const float *__re
 
 	// acc[t][h][r] = partial for a = a0 + 4*(4g + r) + t, c = 16h + j.
 	if constexpr (NB > 1) {
-		__shared__ float red[NB - 1][NA][4][H][4][64];
+		float (*red)[NA][4][H][4][64] = reinterpret_cast<float (*)[NA][4][H][4][64]>(smem);
 		if (wb > 0) {
 #pragma unroll
 			for (int t = 0; t < 4; t++)
@@ -592,7 +628,7 @@ __global__ __launch_bounds__(64 * NA * NB) void k_partial_mfma(const float *__re
 	}
 	if (idle)
 		return;
-	float *dst = part + (size_t)blockIdx.y * p * KT;
+	float *dst = ha.part + (size_t)by * p * KT;
 #pragma unroll
 	for (int t = 0; t < 4; t++)
 #pragma unroll
@@ -605,38 +641,81 @@ __global__ __launch_bounds__(64 * NA * NB) void k_partial_mfma(const float *__re
 		}
 }
 
-// A[a][c] = (A[a][c] * sum_s part[s][a][c]) / den[c], optional clamp, then the column sums
-// of the new A per block of 64 rows (the next half-step's denominators).  Kernel_mul_div
-// order: multiply first, then divide (src/GPU_kernels.cu:2029-2033).
-template <int KT>
-__global__ __launch_bounds__(64 * fin_tpr(KT)) void k_finish(const float *__restrict__ part, int S,
-							      float *__restrict__ A, int p, int k,
-							      const float *__restrict__ den,
-							      float *__restrict__ colA, int clamp,
-							      const nmfk_state *__restrict__ st)
+template <int KT, int NA, int NB>
+__global__ __launch_bounds__(64 * NA * NB) void k_partial_mfma(half_args h, const nmfk_state *__restrict__ st)
 {
+	__shared__ float smem[smem_mfma(KT, NA, NB)];
 	if (st != nullptr && st->converged)
 		return;
+	partial_mfma_body<KT, NA, NB>(blockIdx.x, blockIdx.y, h, smem);
+}
+
+// A[a][c] = (A[a][c] * sum_s part[s][a][c]) / den[c], optional clamp, then the column sums
+// of the new A per block of 64 rows (the next half-step's denominators).  Kernel_mul_div
+// order: multiply first, then divide (src/GPU_kernels.cu:2029-2033).  Block bx covers rows
+// [64 bx, 64 bx + 64); smem holds NMFK_FIN_ROWS x (KT + 1) floats.
+__host__ __device__ constexpr int smem_fin(int kt)
+{
+	return NMFK_FIN_ROWS * (kt + 1);
+}
+
+// Q consecutive floats at a Q-aligned float offset (Q = KT/4: 1, 2, or a multiple of 4).
+template <int Q>
+__device__ __forceinline__ void ldq(const float *__restrict__ p, float (&v)[Q])
+{
+	if constexpr (Q % 4 == 0) {
+#pragma unroll
+		for (int i = 0; i < Q; i += 4) {
+			const float4 t = *reinterpret_cast<const float4 *>(p + i);
+			v[i] = t.x, v[i + 1] = t.y, v[i + 2] = t.z, v[i + 3] = t.w;
+		}
+	} else if constexpr (Q == 2) {
+		const float2 t = *reinterpret_cast<const float2 *>(p);
+		v[0] = t.x, v[1] = t.y;
+	} else {
+#pragma unroll
+		for (int i = 0; i < Q; i++)
+			v[i] = p[i];
+	}
+}
+
+template <int Q>
+__device__ __forceinline__ void stq(float *__restrict__ p, const float (&v)[Q])
+{
+	if constexpr (Q % 4 == 0) {
+#pragma unroll
+		for (int i = 0; i < Q; i += 4)
+			*reinterpret_cast<float4 *>(p + i) = make_float4(v[i], v[i + 1], v[i + 2], v[i + 3]);
+	} else if constexpr (Q == 2) {
+		*reinterpret_cast<float2 *>(p) = make_float2(v[0], v[1]);
+	} else {
+#pragma unroll
+		for (int i = 0; i < Q; i++)
+			p[i] = v[i];
+	}
+}
+
+template <int KT>
+__device__ __forceinline__ void finish_body(int bx, const float *__restrict__ part, int S, float *__restrict__ A,
+					    int p, int k, const float *__restrict__ den, float *__restrict__ colA,
+					    int clamp, float *__restrict__ smem)
+{
 	constexpr int TPR = fin_tpr(KT), Q = fin_q(KT);
-	const int a = blockIdx.x * NMFK_FIN_ROWS + threadIdx.x / TPR;
+	const int a = bx * NMFK_FIN_ROWS + threadIdx.x / TPR;
 	const int c0 = (threadIdx.x % TPR) * Q;
 	float x[Q];
 	if (a < p) {
 		const size_t pst = (size_t)p * KT;
 		const float *pa = part + (size_t)a * KT + c0;
 		float s[Q];
-#pragma unroll
-		for (int i = 0; i < Q; i++)
-			s[i] = pa[i];
+		ldq<Q>(pa, s);
 		int sp = 1;
 		// 8 splits' loads in flight per step; the additions keep ascending split order.
 		for (; sp + 8 <= S; sp += 8) {
 			float v[8][Q];
 #pragma unroll
 			for (int u = 0; u < 8; u++)
-#pragma unroll
-				for (int i = 0; i < Q; i++)
-					v[u][i] = pa[(sp + u) * pst + i];
+				ldq<Q>(pa + (sp + u) * pst, v[u]);
 #pragma unroll
 			for (int u = 0; u < 8; u++)
 #pragma unroll
@@ -644,38 +723,52 @@ __global__ __launch_bounds__(64 * fin_tpr(KT)) void k_finish(const float *__rest
 					s[i] += v[u][i];
 		}
 		for (; sp < S; sp++) {
-			const float *ps = pa + sp * pst;
+			float v[Q];
+			ldq<Q>(pa + sp * pst, v);
 #pragma unroll
 			for (int i = 0; i < Q; i++)
-				s[i] += ps[i];
+				s[i] += v[i];
 		}
 		float *Ar = A + (size_t)a * KT + c0;
+		float ar[Q];
+		ldq<Q>(Ar, ar);
 #pragma unroll
 		for (int i = 0; i < Q; i++) {
 			float v = 0.0f;
 			if (c0 + i < k) {
-				v = Ar[i] * s[i];
+				v = ar[i] * s[i];
 				v = v / den[c0 + i];
 				if (clamp)
 					v = clamp_rmin(v);
 			}
 			x[i] = v;
 		}
-#pragma unroll
-		for (int i = 0; i < Q; i++)
-			Ar[i] = x[i];
+		stq<Q>(Ar, x);
 	} else {
 #pragma unroll
 		for (int i = 0; i < Q; i++)
 			x[i] = 0.0f;
 	}
-	block_colsum_store<KT>(x, colA + (size_t)blockIdx.x * KT);
+	block_colsum_store<KT>(x, colA + (size_t)bx * KT, smem);
 }
 
 template <int KT>
-__global__ __launch_bounds__(64 * fin_tpr(KT)) void k_colsum_parts(const float *__restrict__ A, int p,
-								    float *__restrict__ colA)
+__global__ __launch_bounds__(NMFK_WG) void k_finish(const float *__restrict__ part, int S, float *__restrict__ A,
+						    int p, int k, const float *__restrict__ den,
+						    float *__restrict__ colA, int clamp,
+						    const nmfk_state *__restrict__ st)
 {
+	__shared__ float smem[smem_fin(KT)];
+	if (st != nullptr && st->converged)
+		return;
+	finish_body<KT>(blockIdx.x, part, S, A, p, k, den, colA, clamp, smem);
+}
+
+template <int KT>
+__global__ __launch_bounds__(NMFK_WG) void k_colsum_parts(const float *__restrict__ A, int p,
+							  float *__restrict__ colA)
+{
+	__shared__ float smem[smem_fin(KT)];
 	constexpr int TPR = fin_tpr(KT), Q = fin_q(KT);
 	const int a = blockIdx.x * NMFK_FIN_ROWS + threadIdx.x / TPR;
 	const int c0 = (threadIdx.x % TPR) * Q;
@@ -683,13 +776,14 @@ __global__ __launch_bounds__(64 * fin_tpr(KT)) void k_colsum_parts(const float *
 #pragma unroll
 	for (int i = 0; i < Q; i++)
 		x[i] = (a < p) ? A[(size_t)a * KT + c0 + i] : 0.0f;
-	block_colsum_store<KT>(x, colA + (size_t)blockIdx.x * KT);
+	block_colsum_store<KT>(x, colA + (size_t)blockIdx.x * KT, smem);
 }
 
 template <int KT>
-__global__ __launch_bounds__(256) void k_den(const float *__restrict__ parts, int nparts, float *__restrict__ den)
+__global__ __launch_bounds__(NMFK_WG) void k_den(const float *__restrict__ parts, int nparts, float *__restrict__ den)
 {
-	den_reduce<KT>(parts, nparts, den);
+	__shared__ float smem[4 * KT];
+	den_reduce<KT>(parts, nparts, den, smem);
 }
 
 // --------------------------------------------------------------------------------------
@@ -770,78 +864,93 @@ __global__ __launch_bounds__(256) void k_idx_max(const float *__restrict__ A, in
 
 // --------------------------------------------------------------------------------------
 // Convergence test: adjust(H) + classification + connectivity difference + stop rule
-// (src/NMF_GPU.c:611-708).  One workgroup of 1024 threads; one wave per row of H.
-// The difference count  #{i<j : [c_i==c_j] != [l_i==l_j]}  equals
-//   sum_a C(r_a,2) + sum_b C(s_b,2) - 2 sum_ab C(n_ab,2)
-// over the contingency table n_ab = #{j : c_j == a, l_j == b}; integer and exact.
+// (src/NMF_GPU.c:611-708), as two passes.
+//   rows pass: each block clamps NMFK_TEST_ROWS rows of H (written back) and classifies
+//              them (8 rows per wave, loaded together) into cls;
+//   final pass (one block per job): the contingency table n_ab = #{j : c_j == a, l_j == b}
+//              in LDS, then the difference  #{i<j : [c_i==c_j] != [l_i==l_j]}  as
+//              sum_a C(r_a,2) + sum_b C(s_b,2) - 2 sum_ab C(n_ab,2)  (integer, exact),
+//              last <- cls, and the stop rule.  A job whose test budget (max_tests) is
+//              spent without converging is marked converged = 2 (done, not converged).
 // --------------------------------------------------------------------------------------
 
-__global__ __launch_bounds__(1024) void k_test(float *__restrict__ H, int m, int k, int kt,
-					       uint32_t *__restrict__ cls, uint32_t *__restrict__ last,
-					       nmfk_state *__restrict__ st)
+__device__ __forceinline__ void test_rows_body(int bx, float *__restrict__ H, int m, int k, int kt,
+					       uint32_t *__restrict__ cls)
 {
-	if (st->converged)
-		return;
-	__shared__ uint32_t tab[NMFK_KMAX * NMFK_KMAX];
-	__shared__ unsigned long long red[16];
-	for (int e = threadIdx.x; e < NMFK_KMAX * NMFK_KMAX; e += 1024)
-		tab[e] = 0;
-	__syncthreads();
-	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	constexpr int R = NMFK_TEST_ROWS / (NMFK_WG / 64);
+	const int lane = threadIdx.x & 63;
+	const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
 	const int bw = ref_block_width(k);
-	for (int j = wave; j < m; j += 16) {
-		float *row = H + (size_t)j * kt;
-		float v[3];
-		load_row_items(row, k, bw, lane, v);
+	const int j0 = bx * NMFK_TEST_ROWS + wave * R;
+	float v[R][3];
 #pragma unroll
-		for (int i = 0; i < 3; i++) {
-			const int cc = lane + i * bw;
+	for (int i = 0; i < R; i++) {
+		if (j0 + i < m)
+			load_row_items(H + (size_t)(j0 + i) * kt, k, bw, lane, v[i]);
+	}
+#pragma unroll
+	for (int i = 0; i < R; i++) {
+		const int j = j0 + i;
+		if (j >= m)
+			break;
+		float *row = H + (size_t)j * kt;
+#pragma unroll
+		for (int t = 0; t < 3; t++) {
+			const int cc = lane + t * bw;
 			if (lane < bw && cc < k) {
-				v[i] = clamp_rmin(v[i]);
-				row[cc] = v[i];
+				v[i][t] = clamp_rmin(v[i][t]);
+				row[cc] = v[i][t];
 			}
 		}
-		const int c = wave_argmax_ref(v, k, bw, lane);
-		if (lane == 0) {
-			const uint32_t l = last[j];
+		const int c = wave_argmax_ref(v[i], k, bw, lane);
+		if (lane == 0)
 			cls[j] = (uint32_t)c;
-			last[j] = (uint32_t)c;
-			atomicAdd(&tab[c * NMFK_KMAX + (int)l], 1u);
-		}
+	}
+}
+
+// tab: NMFK_KMAX * NMFK_KMAX uint32 of LDS; red: 2 * (NMFK_WG / 64) uint64 of LDS.
+__device__ __forceinline__ void test_final_body(int m, const uint32_t *__restrict__ cls, uint32_t *__restrict__ last,
+						nmfk_state *__restrict__ st, uint32_t *__restrict__ tab,
+						unsigned long long *__restrict__ red)
+{
+	constexpr int NWV = NMFK_WG / 64;
+	for (int e = threadIdx.x; e < NMFK_KMAX * NMFK_KMAX; e += NMFK_WG)
+		tab[e] = 0;
+	__syncthreads();
+	for (int j = threadIdx.x; j < m; j += NMFK_WG) {
+		const uint32_t c = cls[j], l = last[j];
+		last[j] = c;
+		atomicAdd(&tab[c * NMFK_KMAX + l], 1u);
 	}
 	__syncthreads();
-	// S_both - S_new/2 - S_old/2 accumulated as  2*C(n_ab,2) terms and marginal terms.
-	unsigned long long both = 0, rows = 0, cols = 0;
-	for (int e = threadIdx.x; e < NMFK_KMAX * NMFK_KMAX; e += 1024) {
+	unsigned long long both = 0, t = 0;
+	for (int e = threadIdx.x; e < NMFK_KMAX * NMFK_KMAX; e += NMFK_WG) {
 		const unsigned long long v = tab[e];
 		both += v * (v - (v > 0)) / 2;
 	}
 	if (threadIdx.x < NMFK_KMAX) {
-		unsigned long long r = 0, s = 0;
+		unsigned long long r = 0, c = 0;
 		for (int b = 0; b < NMFK_KMAX; b++) {
 			r += tab[threadIdx.x * NMFK_KMAX + b];
-			s += tab[b * NMFK_KMAX + threadIdx.x];
+			c += tab[b * NMFK_KMAX + threadIdx.x];
 		}
-		rows = r * (r - (r > 0)) / 2;
-		cols = s * (s - (s > 0)) / 2;
+		t = r * (r - (r > 0)) / 2 + c * (c - (c > 0)) / 2;
 	}
-	// exact integer reductions (order-free)
-	unsigned long long t = rows + cols;
 	for (int off = 32; off >= 1; off >>= 1) {
 		both += __shfl_xor(both, off, 64);
 		t += __shfl_xor(t, off, 64);
 	}
-	__shared__ unsigned long long red2[16];
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 	if (lane == 0) {
 		red[wave] = both;
-		red2[wave] = t;
+		red[NWV + wave] = t;
 	}
 	__syncthreads();
 	if (threadIdx.x == 0) {
 		unsigned long long B = 0, Tm = 0;
-		for (int w = 0; w < 16; w++) {
+		for (int w = 0; w < NWV; w++) {
 			B += red[w];
-			Tm += red2[w];
+			Tm += red[NWV + w];
 		}
 		const unsigned long long diff = Tm - 2ull * B;
 		st->last_diff = diff;
@@ -854,10 +963,174 @@ __global__ __launch_bounds__(1024) void k_test(float *__restrict__ H, int m, int
 			st->converged = 1;
 			st->stop_test = st->ntests;
 		}
+		if (!st->converged && st->ntests >= st->max_tests)
+			st->converged = 2;
 	}
 }
 
-__global__ void k_state_reset(nmfk_state *st, uint32_t stop_threshold, uint32_t *last, int m)
+__global__ __launch_bounds__(NMFK_WG) void k_test_rows(float *__restrict__ H, int m, int k, int kt,
+						       uint32_t *__restrict__ cls, const nmfk_state *__restrict__ st)
+{
+	if (st->converged)
+		return;
+	test_rows_body(blockIdx.x, H, m, k, kt, cls);
+}
+
+__global__ __launch_bounds__(NMFK_WG) void k_test_final(int m, const uint32_t *__restrict__ cls,
+							uint32_t *__restrict__ last, nmfk_state *__restrict__ st)
+{
+	__shared__ uint32_t tab[NMFK_KMAX * NMFK_KMAX];
+	__shared__ unsigned long long red[2 * (NMFK_WG / 64)];
+	if (st->converged)
+		return;
+	test_final_body(m, cls, last, st, tab, red);
+}
+
+// --------------------------------------------------------------------------------------
+// Batched launches (continuous batching and rank sweeps): one grid serves every active
+// job.  Workgroups walk a tile table; a tile is one workgroup (bx, by) of the job's own
+// single-job grid, run by the same body, so the job's arithmetic does not depend on what
+// else is in the batch.  Bodies: VALU <KT, 4, 4> for KT 4/8, MFMA <KT, 1, 4> for KT 16/32
+// (the single-job planner's defaults; the host checks every job's plan before batching).
+// --------------------------------------------------------------------------------------
+
+__device__ __forceinline__ half_args batch_half(const nmfk_bjob &J, int half)
+{
+	half_args h;
+	const bool hh = half == 0;
+	const nmfk_hplan &pl = hh ? J.planH : J.planW;
+	h.U = hh ? J.V : J.Vt;
+	h.ldu = hh ? J.ldv : J.ldvt;
+	h.A = hh ? J.H : J.W;
+	h.B = hh ? J.W : J.H;
+	h.p = hh ? J.m : J.n;
+	h.q = hh ? J.n : J.m;
+	h.bchunk = pl.bchunk;
+	h.nab = pl.nab;
+	h.part = J.part;
+	h.colB = hh ? J.colW : J.colH;
+	h.ncolB = hh ? J.ncolW : J.ncolH;
+	h.den = hh ? J.denW : J.denH;
+	return h;
+}
+
+constexpr int smem_batch_partial()
+{
+	return smem_valu(8, 4, 4) > smem_mfma(NMFK_BATCH_KTMAX, 1, 4) ? smem_valu(8, 4, 4)
+								       : smem_mfma(NMFK_BATCH_KTMAX, 1, 4);
+}
+
+__global__ __launch_bounds__(NMFK_WG) void k_batch_partial(const nmfk_bjob *__restrict__ jobs,
+							   const nmfk_tile *__restrict__ tiles, int ntiles, int half)
+{
+	__shared__ float smem[smem_batch_partial()];
+	for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+		const nmfk_tile tl = tiles[t];
+		const nmfk_bjob &J = jobs[tl.job];
+		if (!J.live || J.st->converged)
+			continue;
+		const half_args h = batch_half(J, half);
+		switch (J.kt) {
+		case 4:
+			partial_valu_body<4, 4, 4>(tl.bx, tl.by, h, smem);
+			break;
+		case 8:
+			partial_valu_body<8, 4, 4>(tl.bx, tl.by, h, smem);
+			break;
+		case 16:
+			partial_mfma_body<16, 1, 4>(tl.bx, tl.by, h, smem);
+			break;
+		case 32:
+			partial_mfma_body<32, 1, 4>(tl.bx, tl.by, h, smem);
+			break;
+		}
+		__syncthreads();   // the next tile reuses the LDS
+	}
+}
+
+__global__ __launch_bounds__(NMFK_WG) void k_batch_finish(const nmfk_bjob *__restrict__ jobs,
+							  const nmfk_tile *__restrict__ tiles, int ntiles, int half,
+							  int clamp)
+{
+	__shared__ float smem[smem_fin(NMFK_BATCH_KTMAX)];
+	for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+		const nmfk_tile tl = tiles[t];
+		const nmfk_bjob &J = jobs[tl.job];
+		if (!J.live || J.st->converged)
+			continue;
+		const bool hh = half == 0;
+		const float *part = J.part;
+		const int S = hh ? J.planH.S : J.planW.S;
+		float *A = hh ? J.H : J.W;
+		const int p = hh ? J.m : J.n;
+		const float *den = hh ? J.denW : J.denH;
+		float *colA = hh ? J.colH : J.colW;
+		const int cl = hh ? 0 : clamp;
+		switch (J.kt) {
+		case 4:
+			finish_body<4>(tl.bx, part, S, A, p, J.k, den, colA, cl, smem);
+			break;
+		case 8:
+			finish_body<8>(tl.bx, part, S, A, p, J.k, den, colA, cl, smem);
+			break;
+		case 16:
+			finish_body<16>(tl.bx, part, S, A, p, J.k, den, colA, cl, smem);
+			break;
+		case 32:
+			finish_body<32>(tl.bx, part, S, A, p, J.k, den, colA, cl, smem);
+			break;
+		}
+		__syncthreads();
+	}
+}
+
+__global__ __launch_bounds__(NMFK_WG) void k_batch_test_rows(const nmfk_bjob *__restrict__ jobs,
+							     const nmfk_tile *__restrict__ tiles, int ntiles)
+{
+	for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+		const nmfk_tile tl = tiles[t];
+		const nmfk_bjob &J = jobs[tl.job];
+		if (!J.live || J.st->converged)
+			continue;
+		test_rows_body(tl.bx, J.H, J.m, J.k, J.kt, J.cls);
+	}
+}
+
+__global__ __launch_bounds__(NMFK_WG) void k_batch_test_final(const nmfk_bjob *__restrict__ jobs, int njobs)
+{
+	__shared__ uint32_t tab[NMFK_KMAX * NMFK_KMAX];
+	__shared__ unsigned long long red[2 * (NMFK_WG / 64)];
+	for (int t = blockIdx.x; t < njobs; t += gridDim.x) {
+		const nmfk_bjob &J = jobs[t];
+		if (!J.live || J.st->converged)
+			continue;
+		test_final_body(J.m, J.cls, J.last, J.st, tab, red);
+		__syncthreads();
+	}
+}
+
+// out[t] = state of job t (zeros for an empty slot), for one host copy per poll.
+__global__ void k_batch_gather_states(const nmfk_bjob *__restrict__ jobs, int njobs, nmfk_state *__restrict__ out)
+{
+	const int t = blockIdx.x * blockDim.x + threadIdx.x;
+	if (t < njobs) {
+		nmfk_state s = {};
+		if (jobs[t].live)
+			s = *jobs[t].st;
+		out[t] = s;
+	}
+}
+
+// C[i][j] += (cls[i] == cls[j]); integer adds, so the order of jobs does not matter.
+__global__ __launch_bounds__(256) void k_consensus_add(const uint32_t *__restrict__ cls, int m, int32_t *__restrict__ C)
+{
+	const int j = blockIdx.x * 64 + (threadIdx.x & 63);
+	const int i = blockIdx.y * 4 + (threadIdx.x >> 6);
+	if (i < m && j < m)
+		C[(size_t)i * m + j] += (cls[i] == cls[j]) ? 1 : 0;
+}
+
+__global__ void k_state_reset(nmfk_state *st, uint32_t stop_threshold, uint32_t max_tests, uint32_t *last, int m)
 {
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i == 0) {
@@ -867,7 +1140,7 @@ __global__ void k_state_reset(nmfk_state *st, uint32_t stop_threshold, uint32_t 
 		st->stop_test = 0;
 		st->last_diff = 0;
 		st->stop_threshold = stop_threshold;
-		st->pad_ = 0;
+		st->max_tests = max_tests;
 	}
 	for (int j = i; j < m; j += gridDim.x * blockDim.x)
 		last[j] = 0;
@@ -1110,6 +1383,12 @@ extern "C" int nmfk_tune_get(int key)
 
 extern "C" void nmfk_plan_half_path(uint32_t p, uint32_t q, uint32_t kt, int path, nmfk_hplan *pl)
 {
+	nmfk_plan_half_target(p, q, kt, path, 0, pl);
+}
+
+extern "C" void nmfk_plan_half_target(uint32_t p, uint32_t q, uint32_t kt, int path, uint32_t target_wgs,
+				      nmfk_hplan *pl)
+{
 	if (path == NMFK_PATH_AUTO)
 		path = (kt % 16 == 0) ? NMFK_PATH_MFMA32 : NMFK_PATH_VALU;
 	if (path != NMFK_PATH_VALU && kt % 16 != 0)
@@ -1121,11 +1400,12 @@ extern "C" void nmfk_plan_half_path(uint32_t p, uint32_t q, uint32_t kt, int pat
 		na = 1, nb = 4;
 	const bool valu = path == NMFK_PATH_VALU;
 	const uint32_t ap = valu ? default_ap(kt) : (uint32_t)na;
-	const uint32_t nw = valu ? 8u : (uint32_t)(na * nb);
+	const uint32_t nw = valu ? 4u : (uint32_t)(na * nb);
 	const uint32_t rows_per_step = valu ? 1u : 16u;           // rows of U per wave step
 	const uint32_t steppers = valu ? nw : (uint32_t)nb;        // wave groups splitting the rows
 	const uint32_t nab = (p + 64u * ap - 1u) / (64u * ap);
-	const uint32_t target = (uint32_t)g_tune[valu ? NMFK_TUNE_WGS_VALU : NMFK_TUNE_WGS_MFMA];
+	const uint32_t target =
+		target_wgs ? target_wgs : (uint32_t)g_tune[valu ? NMFK_TUNE_WGS_VALU : NMFK_TUNE_WGS_MFMA];
 	uint32_t S = (target + nab - 1u) / nab;
 	const uint32_t min_rows = steppers * rows_per_step * (uint32_t)g_tune[NMFK_TUNE_MIN_STEPS];
 	const uint32_t smax = (q + min_rows - 1u) / min_rows;
@@ -1157,10 +1437,10 @@ static int launch_partial_t(const float *U, size_t ldu, const float *A, const fl
 			    float *den, const nmfk_state *st, hipStream_t stream)
 {
 	dim3 grid(pl->nab + 1, pl->S);   // + the denominator column
-	if (pl->nw != 8)
+	if (pl->nw != 4)
 		return (int)hipErrorInvalidValue;
-	hipLaunchKernelGGL((k_partial<KT, AP, 8>), grid, dim3(512), 0, stream, U, ldu, A, B, (int)p,
-			   (int)q, (int)pl->bchunk, part, colB, (int)ncolB, den, st);
+	const half_args h = {U, ldu, A, B, (int)p, (int)q, (int)pl->bchunk, (int)pl->nab, part, colB, (int)ncolB, den};
+	hipLaunchKernelGGL((k_partial<KT, AP, 4>), grid, dim3(256), 0, stream, h, st);
 	return (int)hipGetLastError();
 }
 
@@ -1173,10 +1453,10 @@ static int launch_partial_mfma_t(const float *U, size_t ldu, const float *A, con
 	const int na = (int)pl->ap, nb = pl->ap ? (int)(pl->nw / pl->ap) : 0;
 	if (pl->bchunk % 16 != 0 || !mfma_shape_ok(na, nb) || (size_t)q * ldu * 4 > 0x7fffffffull)
 		return (int)hipErrorInvalidValue;
+	const half_args h = {U, ldu, A, B, (int)p, (int)q, (int)pl->bchunk, (int)pl->nab, part, colB, (int)ncolB, den};
 #define NMFK_MFMA_SHAPE(NA_, NB_)                                                                        \
 	if (na == NA_ && nb == NB_) {                                                                  \
-		hipLaunchKernelGGL((k_partial_mfma<KT, NA_, NB_>), grid, dim3(64 * NA_ * NB_), 0, stream, U, ldu, \
-				   A, B, (int)p, (int)q, (int)pl->bchunk, part, colB, (int)ncolB, den, st);   \
+		hipLaunchKernelGGL((k_partial_mfma<KT, NA_, NB_>), grid, dim3(64 * NA_ * NB_), 0, stream, h, st); \
 		return (int)hipGetLastError();                                                         \
 	}
 	NMFK_MFMA_SHAPE(1, 4)
@@ -1228,7 +1508,7 @@ extern "C" int nmfk_launch_finish(const float *part, uint32_t S, float *A, uint3
 	switch (kt) {
 #define X(K)                                                                                           \
 	case K:                                                                                        \
-		hipLaunchKernelGGL((k_finish<K>), dim3(g), dim3(64 * fin_tpr(K)), 0, stream, part, (int)S, A, (int)p, \
+		hipLaunchKernelGGL((k_finish<K>), dim3(g), dim3(NMFK_WG), 0, stream, part, (int)S, A, (int)p, \
 				   (int)k, den, colA, clamp, st);                                       \
 		return (int)hipGetLastError();
 		NMFK_KT_LIST(X)
@@ -1244,7 +1524,7 @@ extern "C" int nmfk_launch_colsum_parts(const float *A, uint32_t p, uint32_t kt,
 	switch (kt) {
 #define X(K)                                                                                      \
 	case K:                                                                                   \
-		hipLaunchKernelGGL((k_colsum_parts<K>), dim3(g), dim3(64 * fin_tpr(K)), 0, stream, A, (int)p, colA); \
+		hipLaunchKernelGGL((k_colsum_parts<K>), dim3(g), dim3(NMFK_WG), 0, stream, A, (int)p, colA); \
 		return (int)hipGetLastError();
 		NMFK_KT_LIST(X)
 #undef X
@@ -1272,15 +1552,73 @@ extern "C" int nmfk_launch_test(float *H, uint32_t m, uint32_t k, uint32_t kt, u
 {
 	if (k > NMFK_KMAX || k == 0)
 		return (int)hipErrorInvalidValue;
-	hipLaunchKernelGGL(k_test, dim3(1), dim3(1024), 0, stream, H, (int)m, (int)k, (int)kt, cls, last, st);
+	const unsigned g = (m + NMFK_TEST_ROWS - 1) / NMFK_TEST_ROWS;
+	hipLaunchKernelGGL(k_test_rows, dim3(g), dim3(NMFK_WG), 0, stream, H, (int)m, (int)k, (int)kt, cls,
+			   (const nmfk_state *)st);
+	hipLaunchKernelGGL(k_test_final, dim3(1), dim3(NMFK_WG), 0, stream, (int)m, (const uint32_t *)cls, last, st);
 	return (int)hipGetLastError();
 }
 
-extern "C" int nmfk_launch_state_reset(nmfk_state *st, uint32_t stop_threshold, uint32_t *last, uint32_t m,
-				       hipStream_t stream)
+static inline unsigned batch_grid(uint32_t ntiles, uint32_t grid)
 {
-	hipLaunchKernelGGL(k_state_reset, dim3(grid1d(m, 256)), dim3(256), 0, stream, st, stop_threshold, last,
-			   (int)m);
+	if (grid == 0 || grid > ntiles)
+		grid = ntiles;
+	return grid ? grid : 1;
+}
+
+extern "C" int nmfk_launch_batch_partial(const nmfk_bjob *jobs, const nmfk_tile *tiles, uint32_t ntiles, int half,
+					 uint32_t grid, hipStream_t stream)
+{
+	if (ntiles == 0)
+		return 0;
+	hipLaunchKernelGGL(k_batch_partial, dim3(batch_grid(ntiles, grid)), dim3(NMFK_WG), 0, stream, jobs, tiles,
+			   (int)ntiles, half);
+	return (int)hipGetLastError();
+}
+
+extern "C" int nmfk_launch_batch_finish(const nmfk_bjob *jobs, const nmfk_tile *tiles, uint32_t ntiles, int half,
+					int clamp, uint32_t grid, hipStream_t stream)
+{
+	if (ntiles == 0)
+		return 0;
+	hipLaunchKernelGGL(k_batch_finish, dim3(batch_grid(ntiles, grid)), dim3(NMFK_WG), 0, stream, jobs, tiles,
+			   (int)ntiles, half, clamp);
+	return (int)hipGetLastError();
+}
+
+extern "C" int nmfk_launch_batch_test(const nmfk_bjob *jobs, const nmfk_tile *row_tiles, uint32_t nrow_tiles,
+				      uint32_t njobs, uint32_t grid, hipStream_t stream)
+{
+	if (njobs == 0)
+		return 0;
+	hipLaunchKernelGGL(k_batch_test_rows, dim3(batch_grid(nrow_tiles, grid)), dim3(NMFK_WG), 0, stream, jobs,
+			   row_tiles, (int)nrow_tiles);
+	hipLaunchKernelGGL(k_batch_test_final, dim3(njobs), dim3(NMFK_WG), 0, stream, jobs, (int)njobs);
+	return (int)hipGetLastError();
+}
+
+extern "C" int nmfk_launch_batch_gather_states(const nmfk_bjob *jobs, uint32_t njobs, nmfk_state *out,
+					       hipStream_t stream)
+{
+	if (njobs == 0)
+		return 0;
+	hipLaunchKernelGGL(k_batch_gather_states, dim3((njobs + 255) / 256), dim3(256), 0, stream, jobs, (int)njobs, out);
+	return (int)hipGetLastError();
+}
+
+extern "C" int nmfk_launch_consensus_add(const uint32_t *cls, uint32_t m, int32_t *C, hipStream_t stream)
+{
+	if (m == 0)
+		return 0;
+	hipLaunchKernelGGL(k_consensus_add, dim3((m + 63) / 64, (m + 3) / 4), dim3(256), 0, stream, cls, (int)m, C);
+	return (int)hipGetLastError();
+}
+
+extern "C" int nmfk_launch_state_reset(nmfk_state *st, uint32_t stop_threshold, uint32_t max_tests, uint32_t *last,
+				       uint32_t m, hipStream_t stream)
+{
+	hipLaunchKernelGGL(k_state_reset, dim3(grid1d(m, 256)), dim3(256), 0, stream, st, stop_threshold, max_tests,
+			   last, (int)m);
 	return (int)hipGetLastError();
 }
 

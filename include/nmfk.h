@@ -113,6 +113,90 @@ int nmfk_job_kernel_path(const nmfk_job *job);
 int nmfk_tune_set(int key, int value);
 int nmfk_tune_get(int key);
 
+/* A job of rank k on the V of <src>, without copying it: src must outlive the job, and
+ * its V must not change while the job uses it (nmfk_job_set_V* on the new job fail). */
+nmfk_job *nmfk_job_create_shared(nmfk_job *src, uint32_t k);
+
+/* Re-plans both half-steps aiming for <target_wgs> partial-kernel workgroups (0 = the
+ * process tunables); the number of splits of the contraction range, and with it the
+ * summation order, follows the plan, so jobs compared bit for bit must share plans. */
+int nmfk_job_replan(nmfk_job *job, uint32_t target_wgs);
+/* {nab, S, bchunk, path} of the H half, then of the W half. */
+int nmfk_job_plan_get(const nmfk_job *job, uint32_t out[8]);
+
+/* ---------------------------------------------------------------------------------------
+ * Batched execution (continuous batching).  A batch advances up to <capacity> jobs of
+ * uneven shape (n, m, k <= 32 on the default kernel paths) with one launch per phase of
+ * the iteration; each job's results are bitwise those of nmfk_job_run on it.  Jobs join
+ * with nmfk_batch_add (W and H already initialised) and leave through nmfk_batch_poll
+ * once the stop rule is met or nIters is reached; after poll returns a job, the batch no
+ * longer touches it.  All jobs of a batch share niter_test_conv.  Not thread-safe.
+ * ------------------------------------------------------------------------------------- */
+typedef struct nmfk_batch nmfk_batch;
+
+/* Launch tuning of a batch (nmfk_batch_autotune picks the first two fields). */
+typedef struct nmfk_batch_tuning {
+	uint32_t grid_partial;    /* workgroups of the partial launches (0: one per tile, else each
+	                             workgroup runs ceil(tiles / grid) tiles of possibly different jobs) */
+	uint32_t grid_finish;     /* same for the finish and test launches */
+	uint32_t order;           /* tile order: 0 job by job, 1 split by split across jobs */
+	uint32_t chunks_per_poll; /* test intervals enqueued per nmfk_batch_step in nmfk_batch_run-style loops */
+	uint32_t compact_pct;     /* rebuild the tile tables when finished jobs hold this % of tiles */
+} nmfk_batch_tuning;
+
+void nmfk_batch_default_tuning(nmfk_batch_tuning *t);
+nmfk_batch *nmfk_batch_create(uint32_t niter_test_conv, uint32_t capacity);
+void nmfk_batch_destroy(nmfk_batch *b);
+int nmfk_batch_set_tuning(nmfk_batch *b, const nmfk_batch_tuning *t);
+int nmfk_batch_get_tuning(const nmfk_batch *b, nmfk_batch_tuning *t);
+/* Adds a job; resets its stop-rule state.  Returns its slot (>= 0) or -1. */
+int nmfk_batch_add(nmfk_batch *b, nmfk_job *job, const nmfk_options *opt);
+/* Jobs added and not yet returned by nmfk_batch_poll. */
+uint32_t nmfk_batch_active(const nmfk_batch *b);
+/* Enqueues <chunks> test intervals (niter_test_conv iterations + test each) for every
+ * active job; asynchronous. */
+int nmfk_batch_step(nmfk_batch *b, uint32_t chunks);
+/* Waits for the enqueued chunks, runs the leftover nIters % niter_test_conv iterations of
+ * jobs that reached nIters unconverged, and returns (up to <cap>) finished jobs with their
+ * results (seconds = time since nmfk_batch_add). */
+int nmfk_batch_poll(nmfk_batch *b, nmfk_job **done, nmfk_result *res, uint32_t cap, uint32_t *ndone);
+/* Autotuner: times <iters> batched iterations of the current jobs for each candidate
+ * launch shape and keeps the fastest (W and H are restored afterwards). */
+int nmfk_batch_autotune(nmfk_batch *b, uint32_t iters, nmfk_batch_tuning *best, double *best_ms_per_iter);
+
+/* ---------------------------------------------------------------------------------------
+ * Rank sweep x restarts over one V with consensus matrices.
+ * ------------------------------------------------------------------------------------- */
+#define NMFK_SWEEP_BATCHED 0
+#define NMFK_SWEEP_ONE_AT_A_TIME 1
+
+typedef struct nmfk_sweep_config {
+	const uint32_t *ks;       /* ranks */
+	uint32_t nk;
+	uint32_t restarts;        /* restarts per rank; restart r of rank k uses nmfk_sweep_seed(base_seed, k, r) */
+	uint64_t base_seed;
+	uint32_t nIters, niter_test_conv, stop_threshold;
+	uint32_t mode;            /* NMFK_SWEEP_BATCHED or NMFK_SWEEP_ONE_AT_A_TIME */
+	uint32_t plan_target;     /* partial-kernel workgroups each job's plan aims for (0: process tunables) */
+	uint32_t autotune;        /* batched mode: run nmfk_batch_autotune first */
+	uint32_t autotune_iters;
+} nmfk_sweep_config;
+
+typedef struct nmfk_sweep_out {
+	nmfk_result *res;         /* nk * restarts (may be NULL), index q * restarts + r */
+	uint32_t *cls;            /* nk * restarts * m final class vectors (may be NULL) */
+	int32_t *consensus;       /* nk * m * m consensus matrices (may be NULL) */
+	float *W_H_out;           /* per job W (n x k) then H (m x k), jobs in index order (may be NULL) */
+	double seconds;           /* wall time of the factorizations (job set-up and autotuning excluded) */
+	double tune_seconds;
+	nmfk_batch_tuning tuning; /* launch tuning used (batched mode) */
+} nmfk_sweep_out;
+
+void nmfk_sweep_default_config(nmfk_sweep_config *c);
+/* Seed of restart r at rank k: splitmix64 finaliser of base ^ (k << 32 | r). */
+uint64_t nmfk_sweep_seed(uint64_t base, uint32_t k, uint32_t restart);
+int nmfk_sweep_run(nmfk_job *src, const nmfk_sweep_config *cfg, nmfk_sweep_out *out);
+
 /* Raw device pointers and shape (for the drop-in layer and for measurements). */
 typedef struct nmfk_job_info {
 	uint32_t n, m, k, kt;

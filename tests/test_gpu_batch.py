@@ -1,0 +1,184 @@
+"""Batched execution (continuous batching) and the rank sweep, through the C-ABI.
+
+A job advanced inside a batch must give the bits it gives alone (nmfk_job_run on a twin
+job with the same V, seed and kernel plan): W, H, iterations, tests and class vectors.
+The sweep's consensus matrices are checked exactly against the class vectors it returns,
+and sweep jobs against the FP64 oracle (class vectors and stop iteration exact, KL within
+1e-3)."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+
+# (n, m, k, seed_v, seed_init, nIters, stop_threshold): k covers every batched width
+# (kt 4, 8, 16, 32); the last two cannot converge in time and run the leftover
+# nIters % T iterations after their test budget.
+MIX = [
+    (700, 150, 3, 1, 11, 400, 4),
+    (1200, 90, 6, 2, 12, 400, 4),
+    (333, 200, 10, 3, 13, 400, 4),
+    (900, 64, 17, 4, 14, 400, 4),
+    (2000, 300, 32, 5, 15, 400, 4),
+    (64, 20, 2, 6, 16, 400, 4),
+    (500, 77, 5, 7, 17, 123, 1000),
+    (256, 300, 16, 8, 18, 57, 1000),
+]
+T = 5
+
+
+def twins(nmfk, orc, spec):
+    n, m, k, sv, si, _, _ = spec
+    V = orc.gen_lognormal(n, m, sv, 0.0, 1.0)
+    a, b = nmfk.Job(n, m, k), nmfk.Job(n, m, k)
+    for j in (a, b):
+        j.set_V(V)
+        j.init_random(si)
+    return a, b
+
+
+def assert_same(a, b, ra, rb, what):
+    for key in ("iterations", "converged", "ntests", "last_diff"):
+        assert ra[key] == rb[key], (what, key, ra, rb)
+    assert np.array_equal(a.get_W(), b.get_W()), what
+    assert np.array_equal(a.get_H(), b.get_H()), what
+    assert np.array_equal(a.get_classification(), b.get_classification()), what
+
+
+def test_batch_equals_jobs_alone(nmfk, orc):
+    pairs = [twins(nmfk, orc, s) for s in MIX]
+    batch = nmfk.Batch(T, 16)
+    for (a, _), s in zip(pairs, MIX):
+        batch.add(a, s[5], s[6])
+    done = dict((id(j), r) for j, r in batch.run())
+    assert len(done) == len(MIX)
+    for (a, b), s in zip(pairs, MIX):
+        rb = b.run(s[5], T, s[6])
+        assert_same(a, b, done[id(a)], rb, s)
+    # the leftover-iteration jobs really ran past their test budget
+    assert not done[id(pairs[-1][0])]["converged"] and done[id(pairs[-1][0])]["iterations"] == 57
+    for a, b in pairs:
+        a.close(), b.close()
+
+
+def test_batch_admission_mid_run(nmfk, orc):
+    # Jobs arrive while others are running (continuous batching): each result is still the
+    # job's result alone.
+    pairs = [twins(nmfk, orc, s) for s in MIX]
+    batch = nmfk.Batch(T, 4)          # fewer slots than jobs: later jobs wait for a free slot
+    pending = list(range(len(MIX)))
+    results = {}
+    while pending or batch.active:
+        while pending and batch.active < 4:
+            i = pending.pop(0)
+            batch.add(pairs[i][0], MIX[i][5], MIX[i][6])
+        batch.step(1)
+        for j, r in batch.poll():
+            results[id(j)] = r
+    for (a, b), s in zip(pairs, MIX):
+        rb = b.run(s[5], T, s[6])
+        assert_same(a, b, results[id(a)], rb, s)
+    for a, b in pairs:
+        a.close(), b.close()
+
+
+def test_batch_autotune_and_tilings_keep_bits(nmfk, orc):
+    specs = MIX[:5]
+    ref = None
+    for tune in ("autotune", dict(grid_partial=7, grid_finish=3, order=1, compact_pct=0)):
+        pairs = [twins(nmfk, orc, s) for s in specs]
+        batch = nmfk.Batch(T, 8)
+        for (a, _), s in zip(pairs, specs):
+            batch.add(a, s[5], s[6])
+        if tune == "autotune":
+            best, ms = batch.autotune(5)
+            assert ms > 0 and best["order"] in (0, 1)
+        else:
+            batch.set_tuning(**tune)
+        done = dict((id(j), r) for j, r in batch.run())
+        out = [(a.get_W(), a.get_H(), done[id(a)]["iterations"]) for a, _ in pairs]
+        if ref is None:
+            for (a, b), s in zip(pairs, specs):
+                assert_same(a, b, done[id(a)], b.run(s[5], T, s[6]), s)
+            ref = out
+        else:
+            for x, y in zip(out, ref):
+                assert np.array_equal(x[0], y[0]) and np.array_equal(x[1], y[1]) and x[2] == y[2]
+        for a, b in pairs:
+            a.close(), b.close()
+
+
+def test_batch_rejects_bad_jobs(nmfk, orc):
+    batch = nmfk.Batch(T, 2)
+    with nmfk.Job(300, 100, 40) as big:          # kt 48: not a batched width
+        big.init_random(1)
+        with pytest.raises(nmfk.NmfkError):
+            batch.add(big, 100, 4)
+    with nmfk.Job(300, 100, 4) as j:
+        j.init_random(1)
+        with pytest.raises(nmfk.NmfkError):
+            batch.add(j, 3, 4)                   # nIters < niter_test_conv
+
+
+def sweep_input(nmfk, orc, n=600, m=40, kstar=3, seed=21):
+    V = orc.gen_planted(n, m, kstar, seed, 0.3)
+    src = nmfk.Job(n, m, 1)
+    src.set_V(V)
+    return src, V
+
+
+def test_sweep_batched_equals_one_at_a_time(nmfk, orc):
+    src, V = sweep_input(nmfk, orc)
+    ks, R = [2, 3, 4, 5, 9], 3
+    a = nmfk.sweep(src, ks, R, 1234, nIters=600, plan_target=64, want_factors=True)
+    b = nmfk.sweep(src, ks, R, 1234, nIters=600, plan_target=64, mode=nmfk.SWEEP_ONE_AT_A_TIME,
+                   want_factors=True)
+    for ra, rb in zip(a["results"], b["results"]):
+        for key in ("iterations", "converged", "ntests", "last_diff"):
+            assert ra[key] == rb[key]
+    assert np.array_equal(a["cls"], b["cls"])
+    assert np.array_equal(a["consensus"], b["consensus"])
+    for (Wa, Ha), (Wb, Hb) in zip(a["factors"], b["factors"]):
+        assert np.array_equal(Wa, Wb) and np.array_equal(Ha, Hb)
+    # consensus: exact counts from the returned class vectors
+    for q in range(len(ks)):
+        c = a["cls"][q]
+        ref = sum((c[r][:, None] == c[r][None, :]).astype(np.int32) for r in range(R))
+        assert np.array_equal(a["consensus"][q], ref)
+        assert np.all(np.diag(a["consensus"][q]) == R)
+        assert np.array_equal(a["consensus"][q], a["consensus"][q].T)
+    src.close()
+
+
+def test_sweep_jobs_match_oracle(nmfk, orc):
+    src, V = sweep_input(nmfk, orc)
+    ks, R, base = [2, 3], 2, 99
+    a = nmfk.sweep(src, ks, R, base, nIters=2000, autotune=True, autotune_iters=3, want_factors=True)
+    i = 0
+    for k in ks:
+        for r in range(R):
+            W0, H0 = orc.init_WH(V.shape[0], V.shape[1], k, nmfk.sweep_seed(base, k, r))
+            o = orc.nmf(V, W0, H0, 2000, 10, 40, fp64=True)
+            res = a["results"][i]
+            assert res["iterations"] == o["iterations"] and res["converged"] == o["converged"], (k, r)
+            assert np.array_equal(a["cls"][ks.index(k)][r], orc.idx_max(o["H"])), (k, r)
+            W, H = a["factors"][i]
+            assert rel_err(H, o["H"]) < 1e-3 and rel_err(W, o["W"]) < 1e-3, (k, r)
+            kl_gpu, kl_ref = orc.kl_div(V, W, H), orc.kl_div(V, o["W"], o["H"])
+            assert abs(kl_gpu - kl_ref) / kl_ref < 1e-3
+            i += 1
+    src.close()
+
+
+def test_sweep_seed_is_splitmix(nmfk):
+    def splitmix(base, k, r):
+        z = (base ^ ((k << 32) | r)) & (2**64 - 1)
+        z = (z + 0x9E3779B97F4A7C15) & (2**64 - 1)
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & (2**64 - 1)
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & (2**64 - 1)
+        return z ^ (z >> 31)
+    for base, k, r in [(0, 0, 0), (1234, 5, 7), (2**63 + 5, 10, 19)]:
+        assert nmfk.sweep_seed(base, k, r) == splitmix(base, k, r)

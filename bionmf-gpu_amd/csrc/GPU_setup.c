@@ -124,6 +124,12 @@ int setup_GPU(size_t mem_size, bool do_classf)
 		return EXIT_FAILURE;
 	}
 	nmfk_job *j = nmfk_dropin_job;
+	/* The reference computes in FP32 throughout: the drop-in keeps every rank on FP32
+	 * arithmetic (BF16 contractions are opt-in through the job API). */
+	if (j->planH.path == NMFK_PATH_MFMA_BF16 && nmfk_job_set_kernel_path(j, NMFK_PATH_MFMA32)) {
+		print_error(true, "Error: %s\n", nmfk_last_error());
+		return EXIT_FAILURE;
+	}
 	NpP = N, MpP = M, bN = 0, bM = 0;
 	Mp = (index_t)j->ldv;
 	MpPp = Mp;

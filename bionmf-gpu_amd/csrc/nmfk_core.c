@@ -72,6 +72,8 @@ typedef struct job_layout {
 	size_t ldv, ldvt;
 	nmfk_hplan planH, planW;
 	size_t n_V, n_Vt, n_W, n_H, n_part, n_colW, n_colH, n_den, n_dot;
+	uint32_t ldtW, ldtH;
+	size_t n_shadow;               /* uint16 elements of the four BF16 copies (0: none) */
 } job_layout;
 
 static void layout_of(uint32_t n, uint32_t m, uint32_t k, job_layout *L)
@@ -87,7 +89,7 @@ static void layout_of(uint32_t n, uint32_t m, uint32_t k, job_layout *L)
 	L->n_H = (size_t)m * L->kt;
 	/* partial buffer: large enough for every kernel path a job may be switched to */
 	L->n_part = 0;
-	for (int path = NMFK_PATH_VALU; path <= NMFK_PATH_MFMA32; path++) {
+	for (int path = NMFK_PATH_VALU; path <= NMFK_PATH_MFMA_BF16; path++) {
 		nmfk_hplan h, w;
 		nmfk_plan_half_path(m, n, L->kt, path, &h);
 		nmfk_plan_half_path(n, m, L->kt, path, &w);
@@ -100,6 +102,9 @@ static void layout_of(uint32_t n, uint32_t m, uint32_t k, job_layout *L)
 	L->n_colW = (size_t)((n + NMFK_FIN_ROWS - 1) / NMFK_FIN_ROWS) * L->kt;
 	L->n_colH = (size_t)((m + NMFK_FIN_ROWS - 1) / NMFK_FIN_ROWS) * L->kt;
 	L->n_den = 2 * (size_t)L->kt;
+	L->ldtW = (uint32_t)round_up(n, 32);
+	L->ldtH = (uint32_t)round_up(m, 32);
+	L->n_shadow = L->kt % 32 == 0 ? 2 * ((size_t)L->ldtW + L->ldtH) * L->kt : 0;
 	L->n_dot = (size_t)nmfk_dot_kl_blocks(n, m) * 3;
 }
 
@@ -107,7 +112,7 @@ size_t nmfk_job_bytes(uint32_t n, uint32_t m, uint32_t k)
 {
 	job_layout L;
 	layout_of(n, m, k, &L);
-	size_t b = 4 * (L.n_V + L.n_Vt + L.n_W + L.n_H + L.n_part + L.n_colW + L.n_colH + L.n_den);
+	size_t b = 4 * (L.n_V + L.n_Vt + L.n_W + L.n_H + L.n_part + L.n_colW + L.n_colH + L.n_den) + 2 * L.n_shadow;
 	b += 8 * L.n_dot + 4 * 2 * (size_t)m + sizeof(nmfk_state) + 7 * 256;
 	return b;
 }
@@ -183,6 +188,15 @@ static nmfk_job *job_create(uint32_t n, uint32_t m, uint32_t k, nmfk_job *src)
 	if (!e) e |= dalloc((void **)&j->cls, 4 * (size_t)m, "hipMalloc(cls)");
 	if (!e) e |= dalloc((void **)&j->last, 4 * (size_t)m, "hipMalloc(last)");
 	if (!e) e |= dalloc((void **)&j->st, sizeof(nmfk_state), "hipMalloc(state)");
+	j->ldtW = L.ldtW, j->ldtH = L.ldtH;
+	if (!e && L.n_shadow) {
+		e |= dalloc((void **)&j->Wb, 2 * L.n_shadow, "hipMalloc(BF16 copies)");
+		if (!e) {
+			j->Wtb = j->Wb + (size_t)n * L.kt;
+			j->Hb = j->Wtb + (size_t)L.ldtW * L.kt;
+			j->Htb = j->Hb + (size_t)m * L.kt;
+		}
+	}
 	if (!e) e |= nmfk_hip_fail(hipHostMalloc((void **)&j->h_st, 2 * sizeof(nmfk_state), hipHostMallocDefault),
 				   "hipHostMalloc(state)");
 	if (!e) e |= nmfk_hip_fail(hipStreamCreateWithFlags(&j->stream, hipStreamNonBlocking), "hipStreamCreate");
@@ -197,6 +211,8 @@ static nmfk_job *job_create(uint32_t n, uint32_t m, uint32_t k, nmfk_job *src)
 	}
 	/* Zero W and H so their padding columns start (and stay) at zero. */
 	hipMemsetAsync(j->W, 0, 4 * L.n_W, j->stream);
+	if (j->Wb)
+		hipMemsetAsync(j->Wb, 0, 2 * L.n_shadow, j->stream);
 	hipMemsetAsync(j->H, 0, 4 * L.n_H, j->stream);
 	if (j->owns_V) {
 		hipMemsetAsync(j->V, 0, 4 * L.n_V, j->stream);
@@ -221,6 +237,7 @@ void nmfk_job_destroy(nmfk_job *j)
 		hipFree(j->V), hipFree(j->Vt);
 	hipFree(j->W), hipFree(j->H), hipFree(j->part);
 	hipFree(j->colW), hipFree(j->colH), hipFree(j->denW), hipFree(j->dot), hipFree(j->cls), hipFree(j->last);
+	hipFree(j->Wb);
 	hipFree(j->st);
 	if (j->h_st)
 		hipHostFree(j->h_st);
@@ -244,7 +261,7 @@ int nmfk_job_info_get(const nmfk_job *j, nmfk_job_info *info)
 
 int nmfk_job_set_kernel_path(nmfk_job *j, int path)
 {
-	if (path < NMFK_PATH_AUTO || path > NMFK_PATH_MFMA32) {
+	if (path < NMFK_PATH_AUTO || path > NMFK_PATH_MFMA_BF16) {
 		nmfk_set_error("nmfk_job_set_kernel_path: unknown path %d", path);
 		return NMFK_EINVAL;
 	}
@@ -468,32 +485,68 @@ int nmfk_job_get_H(nmfk_job *j, float *H, size_t ldh)
 
 /* ---------------------------------------------------------------- half-steps */
 
+static int partial_H(nmfk_job *j, const nmfk_state *st)
+{
+	if (j->planH.path == NMFK_PATH_MFMA_BF16)
+		return nmfk_launch_partial_bf16(j->V, j->ldv, j->H, j->Wb, j->Wtb, j->ldtW, j->m, j->n, j->kt,
+						&j->planH, j->part, j->colW, j->ncolW, j->denW, st, j->stream);
+	return nmfk_launch_partial(j->V, j->ldv, j->H, j->W, j->m, j->n, j->kt, &j->planH, j->part, j->colW,
+				   j->ncolW, j->denW, st, j->stream);
+}
+
+static int partial_W(nmfk_job *j, const nmfk_state *st)
+{
+	if (j->planW.path == NMFK_PATH_MFMA_BF16)
+		return nmfk_launch_partial_bf16(j->Vt, j->ldvt, j->W, j->Hb, j->Htb, j->ldtH, j->n, j->m, j->kt,
+						&j->planW, j->part, j->colH, j->ncolH, j->denH, st, j->stream);
+	return nmfk_launch_partial(j->Vt, j->ldvt, j->W, j->H, j->n, j->m, j->kt, &j->planW, j->part, j->colH,
+				   j->ncolH, j->denH, st, j->stream);
+}
+
 /* H half: the partial kernel streams V against W (and reduces colsum(W) into denW), the
- * finish kernel applies the update to H and leaves colsum(H) partials for the W half. */
+ * finish kernel applies the update to H and leaves colsum(H) partials for the W half (and
+ * BF16 copies of H when the W half runs in BF16).  On the BF16 path W is read from its
+ * BF16 copies, which the W half's finish (or nmfk_job_shadow_for_H) keeps current. */
 int nmfk_enqueue_half_H(nmfk_job *j, const nmfk_state *st)
 {
-	int e = nmfk_launch_partial(j->V, j->ldv, j->H, j->W, j->m, j->n, j->kt, &j->planH, j->part, j->colW,
-				    j->ncolW, j->denW, st, j->stream);
+	int e = partial_H(j, st);
+	const int sh = j->planW.path == NMFK_PATH_MFMA_BF16;
 	if (!e)
-		e = nmfk_launch_finish(j->part, j->planH.S, j->H, j->m, j->k, j->kt, j->denW, j->colH, 0, st,
-				       j->stream);
+		e = nmfk_launch_finish_shadow(j->part, j->planH.S, j->H, j->m, j->k, j->kt, j->denW, j->colH, 0, st,
+					      sh ? j->Hb : NULL, sh ? j->Htb : NULL, j->ldtH, j->stream);
 	return e;
 }
 
 int nmfk_enqueue_half_W(nmfk_job *j, const nmfk_state *st, int clamp)
 {
-	int e = nmfk_launch_partial(j->Vt, j->ldvt, j->W, j->H, j->n, j->m, j->kt, &j->planW, j->part, j->colH,
-				    j->ncolH, j->denH, st, j->stream);
+	int e = partial_W(j, st);
+	const int sh = j->planH.path == NMFK_PATH_MFMA_BF16;
 	if (!e)
-		e = nmfk_launch_finish(j->part, j->planW.S, j->W, j->n, j->k, j->kt, j->denH, j->colW, clamp, st,
-				       j->stream);
+		e = nmfk_launch_finish_shadow(j->part, j->planW.S, j->W, j->n, j->k, j->kt, j->denH, j->colW, clamp, st,
+					      sh ? j->Wb : NULL, sh ? j->Wtb : NULL, j->ldtW, j->stream);
 	return e;
+}
+
+int nmfk_job_shadow_for_H(nmfk_job *j)
+{
+	if (j->planH.path != NMFK_PATH_MFMA_BF16)
+		return 0;
+	return nmfk_launch_shadow(j->W, j->n, j->kt, j->Wb, j->Wtb, j->ldtW, j->stream);
+}
+
+int nmfk_job_shadow_for_W(nmfk_job *j)
+{
+	if (j->planW.path != NMFK_PATH_MFMA_BF16)
+		return 0;
+	return nmfk_launch_shadow(j->H, j->m, j->kt, j->Hb, j->Htb, j->ldtH, j->stream);
 }
 
 /* update_H(): accum_W = colsum(W) (src/NMF_routines.c:720), then the H half-step. */
 int nmfk_job_update_H(nmfk_job *j)
 {
 	int e = nmfk_launch_colsum_parts(j->W, j->n, j->kt, j->colW, j->stream);
+	if (!e)
+		e = nmfk_job_shadow_for_H(j);
 	if (!e)
 		e = nmfk_enqueue_half_H(j, NULL);
 	return nmfk_hip_fail(e, "update_H");
@@ -505,6 +558,8 @@ int nmfk_job_update_W(nmfk_job *j)
 	if (nmfk_job_refresh_Vt(j))
 		return NMFK_EHIP;
 	int e = nmfk_launch_colsum_parts(j->H, j->m, j->kt, j->colH, j->stream);
+	if (!e)
+		e = nmfk_job_shadow_for_W(j);
 	if (!e)
 		e = nmfk_enqueue_half_W(j, NULL, 0);
 	j->colW_valid = 1;
@@ -584,6 +639,8 @@ int nmfk_job_run(nmfk_job *j, const nmfk_options *opt, nmfk_result *res)
 	int e = nmfk_launch_state_reset(j->st, opt->stop_threshold, UINT32_MAX, j->last, j->m, j->stream);
 	if (!e)
 		e = nmfk_launch_colsum_parts(j->W, j->n, j->kt, j->colW, j->stream);
+	if (!e)
+		e = nmfk_job_shadow_for_H(j);
 	if (e)
 		return nmfk_hip_fail(e, "run setup");
 	if (niter_conv > 0 && (e = build_interval_graph(j, T)) != 0)
@@ -691,16 +748,14 @@ static int time_loop(nmfk_job *j, uint32_t iters, int kind, double *ms)
 				e = nmfk_enqueue_half_W(j, NULL, 0);
 			break;
 		case 1:
-			e = nmfk_launch_partial(j->V, j->ldv, j->H, j->W, j->m, j->n, j->kt, &j->planH, j->part,
-						j->colW, j->ncolW, j->denW, NULL, j->stream);
+			e = partial_H(j, NULL);
 			break;
 		case 2:
 			e = nmfk_launch_finish(j->part, j->planH.S, j->H, j->m, j->k, j->kt, j->denW, j->colH, 0,
 					       NULL, j->stream);
 			break;
 		case 3:
-			e = nmfk_launch_partial(j->Vt, j->ldvt, j->W, j->H, j->n, j->m, j->kt, &j->planW, j->part,
-						j->colH, j->ncolH, j->denH, NULL, j->stream);
+			e = partial_W(j, NULL);
 			break;
 		case 4:
 			e = nmfk_launch_finish(j->part, j->planW.S, j->W, j->n, j->k, j->kt, j->denH, j->colW, 0,
@@ -776,6 +831,10 @@ int nmfk_measure_stream(size_t bytes, uint32_t iters, uint32_t blocks, double *r
 int nmfk_job_time_iterations(nmfk_job *j, uint32_t iters, nmfk_timing *t)
 {
 	int e = nmfk_launch_colsum_parts(j->W, j->n, j->kt, j->colW, j->stream);
+	if (!e)
+		e = nmfk_job_shadow_for_H(j);
+	if (!e)
+		e = nmfk_job_shadow_for_W(j);
 	if (e)
 		return nmfk_hip_fail(e, "timing setup");
 	double ms[5];

@@ -38,7 +38,15 @@ struct nmfk_job {
 	int colW_valid;
 	int vt_stale;                  /* V changed since Vt was built */
 	int owns_V;                    /* 0: V and Vt belong to another job (nmfk_job_create_shared) */
+	/* BF16 copies of W and H for the BF16 kernel path (kt % 32 == 0 jobs only): row-major
+	 * (rows x kt) and transposed (kt x ldt, ldt = rows rounded up to 32, zero padding). */
+	uint16_t *Wb, *Wtb, *Hb, *Htb;
+	uint32_t ldtW, ldtH;
 };
+
+/* Refreshes the BF16 copies of the factor the next half-step reads as B, if its plan is BF16. */
+int nmfk_job_shadow_for_H(nmfk_job *j);
+int nmfk_job_shadow_for_W(nmfk_job *j);
 
 /* Rebuilds Vt from V if V was written since (asynchronous on the job's stream). */
 int nmfk_job_refresh_Vt(nmfk_job *j);

@@ -650,6 +650,237 @@ __global__ __launch_bounds__(64 * NA * NB) void k_partial_mfma(half_args h, cons
 	partial_mfma_body<KT, NA, NB>(blockIdx.x, blockIdx.y, h, smem);
 }
 
+// --------------------------------------------------------------------------------------
+// The half-step with both contractions in BF16 on the matrix cores (MFMA 16x16x32, FP32
+// accumulation); the ratio U / (A . B) and everything after it stay in FP32.
+//
+// Same wave ownership as the FP32 MFMA kernel (4 tiles t of 16 rows a per wave, row
+// a = a0 + 4j + t), but U is walked in 32-row blocks made of two 16-row halves hb:
+//   GEMM1  S[hb][t] = B[bb+16hb .. +16][:] . A[tile t][:]^T   (K = the rank, 32 per MFMA);
+//          lane (g, j) then holds S at rows b = bb + 16hb + 4g + r, r = 0..3;
+//   ratio  R = U / S in FP32 (hardware reciprocal, one product), rounded to BF16;
+//   GEMM2  acc[t][h] += R^T . B   (K = the block's 32 rows): lane (g, j)'s 8 GEMM1 outputs
+//          (both halves) are its A-operand elements, i.e. K element i of lane group g is
+//          row b = bb + 16*(i/4) + 4g + (i%4); the B operand takes B at those same rows.
+// B comes from two BF16 copies the finish kernel keeps beside the FP32 factor: row-major
+// Bb (q x KT, GEMM1: 8 consecutive columns = one 16-byte load) and transposed Btb
+// (KT x ldbt, GEMM2: rows bb+4g..+3 and bb+16+4g..+3 of one column = two 8-byte loads).
+// --------------------------------------------------------------------------------------
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+
+__host__ __device__ constexpr int smem_bf16(int kt, int na, int nb)
+{
+	return smem_mfma(kt, na, nb);
+}
+
+__device__ __forceinline__ bf16x8 to_bf16x8(float4 lo, float4 hi)
+{
+	bf16x8 r;
+	r[0] = (__bf16)lo.x, r[1] = (__bf16)lo.y, r[2] = (__bf16)lo.z, r[3] = (__bf16)lo.w;
+	r[4] = (__bf16)hi.x, r[5] = (__bf16)hi.y, r[6] = (__bf16)hi.z, r[7] = (__bf16)hi.w;
+	return r;
+}
+
+struct bf16_args {
+	const __bf16 *Bb;    // q x KT, row-major
+	const __bf16 *Btb;   // KT x ldbt
+	int ldbt;
+};
+
+template <int KT, int NA, int NB>
+__device__ __forceinline__ void partial_bf16_body(int bx, int by, const half_args &ha, const bf16_args &bb16,
+						  float *__restrict__ smem)
+{
+	static_assert(KT % 32 == 0, "BF16 path needs KT % 32 == 0");
+	constexpr int H = KT / 16;     // 16-column groups (GEMM2 outputs)
+	constexpr int CH = KT / 32;    // 32-column chunks (GEMM1 K steps)
+	const float *__restrict__ U = ha.U;
+	const size_t ldu = ha.ldu;
+	const float *__restrict__ A = ha.A;
+	const __bf16 *__restrict__ Bb = bb16.Bb;
+	const __bf16 *__restrict__ Btb = bb16.Btb;
+	const int ldbt = bb16.ldbt;
+	const int p = ha.p, q = ha.q, bchunk = ha.bchunk;
+	if (bx == ha.nab) {
+		if (by == 0)
+			den_reduce<KT>(ha.colB, ha.ncolB, ha.den, smem);
+		return;
+	}
+	const int lane = threadIdx.x & 63;
+	const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+	const int wa = wave % NA, wb = wave / NA;
+	const int j = lane & 15, g = lane >> 4;
+	const int a0 = (bx * NA + wa) * 64;
+	const bool idle = a0 >= p;
+	const int aj = a0 + 4 * j;
+	const int avail = p - aj;
+
+	// GEMM1 B operand: lane (g, j) supplies A[aj + t][32 ch + 8 g + i] as BF16.
+	bf16x8 aop[4][CH];
+#pragma unroll
+	for (int t = 0; t < 4; t++) {
+		const bool ok = t < avail;
+		const float *Ar = A + (size_t)(ok ? aj + t : 0) * KT + 8 * g;
+#pragma unroll
+		for (int ch = 0; ch < CH; ch++) {
+			float4 lo = *reinterpret_cast<const float4 *>(Ar + 32 * ch);
+			float4 hi = *reinterpret_cast<const float4 *>(Ar + 32 * ch + 4);
+			if (!ok)
+				lo = hi = make_float4(0.f, 0.f, 0.f, 0.f);
+			aop[t][ch] = to_bf16x8(lo, hi);
+		}
+	}
+	floatx4 acc[4][H];
+#pragma unroll
+	for (int t = 0; t < 4; t++)
+#pragma unroll
+		for (int h = 0; h < H; h++)
+			acc[t][h] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+	const int b_begin = by * bchunk;
+	const int b_end = min(q, b_begin + bchunk);
+	const int ucol = avail > 0 ? aj : 0;
+
+	// One 32-row block: u[hb][r] = U[bb + 16hb + 4g + r][aj .. aj+3];
+	// b1[hb][ch] = Bb[bb + 16hb + j][32ch + 8g ..]; b2[h] = Btb[16h + j][rows of K group g].
+	struct blk {
+		float4 u[2][4];
+		bf16x8 b1[2][CH];
+		bf16x8 b2[H];
+	};
+	// Rows at or past b_end read row b_begin (in bounds) and are masked in the ratio.
+	auto load = [&](int bb, blk &x, bool full) {
+#pragma unroll
+		for (int hb = 0; hb < 2; hb++) {
+#pragma unroll
+			for (int r = 0; r < 4; r++) {
+				int b = bb + 16 * hb + 4 * g + r;
+				if (!full)
+					b = b < b_end ? b : b_begin;
+				x.u[hb][r] = *reinterpret_cast<const float4 *>(U + (size_t)b * ldu + ucol);
+			}
+			int b = bb + 16 * hb + j;
+			if (!full)
+				b = b < b_end ? b : b_begin;
+#pragma unroll
+			for (int ch = 0; ch < CH; ch++)
+				x.b1[hb][ch] = *reinterpret_cast<const bf16x8 *>(Bb + (size_t)b * KT + 32 * ch + 8 * g);
+		}
+		// Transposed copy: columns are padded to ldbt >= round_up(q, 32) with zeros, so a
+		// block's 32 rows are always in bounds.
+#pragma unroll
+		for (int h = 0; h < H; h++) {
+			const __bf16 *col = Btb + (size_t)(16 * h + j) * ldbt + bb + 4 * g;
+			const bf16x4 lo = *reinterpret_cast<const bf16x4 *>(col);
+			const bf16x4 hi = *reinterpret_cast<const bf16x4 *>(col + 16);
+			x.b2[h] = bf16x8{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+		}
+	};
+	auto compute = [&](const blk &x, int bb, bool full) {
+		bf16x8 rt[4];
+#pragma unroll
+		for (int hb = 0; hb < 2; hb++) {
+			floatx4 s[4];
+#pragma unroll
+			for (int t = 0; t < 4; t++) {
+				s[t] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+				for (int ch = 0; ch < CH; ch++)
+					s[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(x.b1[hb][ch], aop[t][ch], s[t], 0, 0, 0);
+			}
+#pragma unroll
+			for (int r = 0; r < 4; r++) {
+				const bool rok = full || bb + 16 * hb + 4 * g + r < b_end;
+				const float uv[4] = {x.u[hb][r].x, x.u[hb][r].y, x.u[hb][r].z, x.u[hb][r].w};
+#pragma unroll
+				for (int t = 0; t < 4; t++) {
+					const float v = uv[t] * __builtin_amdgcn_rcpf(s[t][r]);
+					rt[t][4 * hb + r] = (__bf16)(rok ? v : 0.0f);
+				}
+			}
+		}
+#pragma unroll
+		for (int t = 0; t < 4; t++)
+#pragma unroll
+			for (int h = 0; h < H; h++)
+				acc[t][h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(rt[t], x.b2[h], acc[t][h], 0, 0, 0);
+	};
+
+	const int nfull = (b_end - b_begin) / 32;
+	const int mine = idle ? 0 : (nfull > wb ? (nfull - wb + NB - 1) / NB : 0);
+	auto blk_row = [&](int i) { return b_begin + 32 * (wb + NB * min(i, mine - 1)); };
+	if (mine > 0) {
+		// Two-slot ring: the next block's loads are in flight while this one is computed; a
+		// refill past the end re-reads this wave's last block and is never consumed.
+		blk ring[2];
+		load(blk_row(0), ring[0], true);
+		load(blk_row(1), ring[1], true);
+		for (int i = 0; i < mine; i += 2) {
+			compute(ring[0], 0, true);
+			load(blk_row(i + 2), ring[0], true);
+			if (i + 1 < mine) {
+				compute(ring[1], 0, true);
+				load(blk_row(i + 3), ring[1], true);
+			}
+		}
+	}
+	if (!idle && (b_end - b_begin) % 32 != 0 && wb == nfull % NB) {
+		const int bb = b_begin + 32 * nfull;
+		blk x;
+		load(bb, x, false);
+		compute(x, bb, false);
+	}
+
+	if constexpr (NB > 1) {
+		float (*red)[NA][4][H][4][64] = reinterpret_cast<float (*)[NA][4][H][4][64]>(smem);
+		if (wb > 0) {
+#pragma unroll
+			for (int t = 0; t < 4; t++)
+#pragma unroll
+				for (int h = 0; h < H; h++)
+#pragma unroll
+					for (int r = 0; r < 4; r++)
+						red[wb - 1][wa][t][h][r][lane] = acc[t][h][r];
+		}
+		__syncthreads();
+		if (wb > 0 || idle)
+			return;
+#pragma unroll
+		for (int w = 0; w < NB - 1; w++)
+#pragma unroll
+			for (int t = 0; t < 4; t++)
+#pragma unroll
+				for (int h = 0; h < H; h++)
+#pragma unroll
+					for (int r = 0; r < 4; r++)
+						acc[t][h][r] += red[w][wa][t][h][r][lane];
+	}
+	if (idle)
+		return;
+	float *dst = ha.part + (size_t)by * p * KT;
+#pragma unroll
+	for (int t = 0; t < 4; t++)
+#pragma unroll
+		for (int r = 0; r < 4; r++) {
+			const int a = a0 + 4 * (4 * g + r) + t;
+			if (a < p)
+#pragma unroll
+				for (int h = 0; h < H; h++)
+					dst[(size_t)a * KT + 16 * h + j] = acc[t][h][r];
+		}
+}
+
+template <int KT, int NA, int NB>
+__global__ __launch_bounds__(64 * NA * NB) void k_partial_bf16(half_args h, bf16_args b, const nmfk_state *__restrict__ st)
+{
+	__shared__ float smem[smem_bf16(KT, NA, NB)];
+	if (st != nullptr && st->converged)
+		return;
+	partial_bf16_body<KT, NA, NB>(blockIdx.x, blockIdx.y, h, b, smem);
+}
+
 // A[a][c] = (A[a][c] * sum_s part[s][a][c]) / den[c], optional clamp, then the column sums
 // of the new A per block of 64 rows (the next half-step's denominators).  Kernel_mul_div
 // order: multiply first, then divide (src/GPU_kernels.cu:2029-2033).  Block bx covers rows
@@ -695,10 +926,46 @@ __device__ __forceinline__ void stq(float *__restrict__ p, const float (&v)[Q])
 	}
 }
 
+// BF16 copies the finish kernel keeps of the factor it updates (null: none).
+struct shadow_args {
+	__bf16 *Ab;
+	__bf16 *Atb;
+	int ldt;
+};
+
+// BF16 copies of a factor A (p x KT): Ab row-major, Atb = A^T with leading dimension ldt
+// (columns p..ldt-1 stay zero).  Rows [64 bx, 64 bx + 64), 4 threads per row.
+template <int KT>
+__device__ __forceinline__ void store_shadow(int a, int c0, const float (&x)[KT / 4], __bf16 *__restrict__ Ab,
+					     __bf16 *__restrict__ Atb, int ldt)
+{
+	constexpr int Q = KT / 4;
+#pragma unroll
+	for (int i = 0; i < Q; i++) {
+		const __bf16 v = (__bf16)x[i];
+		Ab[(size_t)a * KT + c0 + i] = v;
+		Atb[(size_t)(c0 + i) * ldt + a] = v;
+	}
+}
+
+template <int KT>
+__global__ __launch_bounds__(NMFK_WG) void k_shadow(const float *__restrict__ A, int p, __bf16 *__restrict__ Ab,
+						    __bf16 *__restrict__ Atb, int ldt)
+{
+	constexpr int Q = KT / 4;
+	const int a = blockIdx.x * NMFK_FIN_ROWS + threadIdx.x / 4;
+	const int c0 = (threadIdx.x % 4) * Q;
+	if (a >= p)
+		return;
+	float x[Q];
+	ldq<Q>(A + (size_t)a * KT + c0, x);
+	store_shadow<KT>(a, c0, x, Ab, Atb, ldt);
+}
+
 template <int KT>
 __device__ __forceinline__ void finish_body(int bx, const float *__restrict__ part, int S, float *__restrict__ A,
 					    int p, int k, const float *__restrict__ den, float *__restrict__ colA,
-					    int clamp, float *__restrict__ smem)
+					    int clamp, float *__restrict__ smem, const shadow_args &sh = shadow_args{})
 {
 	constexpr int TPR = fin_tpr(KT), Q = fin_q(KT);
 	const int a = bx * NMFK_FIN_ROWS + threadIdx.x / TPR;
@@ -744,6 +1011,10 @@ __device__ __forceinline__ void finish_body(int bx, const float *__restrict__ pa
 			x[i] = v;
 		}
 		stq<Q>(Ar, x);
+		if constexpr (KT % 32 == 0) {
+			if (sh.Ab != nullptr)
+				store_shadow<KT>(a, c0, x, sh.Ab, sh.Atb, sh.ldt);
+		}
 	} else {
 #pragma unroll
 		for (int i = 0; i < Q; i++)
@@ -756,12 +1027,12 @@ template <int KT>
 __global__ __launch_bounds__(NMFK_WG) void k_finish(const float *__restrict__ part, int S, float *__restrict__ A,
 						    int p, int k, const float *__restrict__ den,
 						    float *__restrict__ colA, int clamp,
-						    const nmfk_state *__restrict__ st)
+						    const nmfk_state *__restrict__ st, shadow_args sh)
 {
 	__shared__ float smem[smem_fin(KT)];
 	if (st != nullptr && st->converged)
 		return;
-	finish_body<KT>(blockIdx.x, part, S, A, p, k, den, colA, clamp, smem);
+	finish_body<KT>(blockIdx.x, part, S, A, p, k, den, colA, clamp, smem, sh);
 }
 
 template <int KT>
@@ -1341,16 +1612,20 @@ static inline unsigned grid1d(size_t total, unsigned block)
 	return (unsigned)(g ? g : 1);
 }
 
-#define NMFK_KT_LIST(X) X(4) X(8) X(16) X(32) X(48) X(64)
-#define NMFK_KT_MFMA_LIST(X) X(16) X(32) X(48) X(64)
+#define NMFK_KT_LIST(X) X(4) X(8) X(16) X(32) X(64)
+#define NMFK_KT_MFMA_LIST(X) X(16) X(32) X(64)
+#define NMFK_KT_BF16_LIST(X) X(32) X(64)
 
 extern "C" uint32_t nmfk_kt(uint32_t k)
 {
-	// k <= 8 stays on the VALU kernel (memory-bound there); larger ranks are padded to the
-	// 16-column granularity of the MFMA kernel.
+	// k <= 8 stays on the VALU kernel (memory-bound there); 9..16 use one 16-column group
+	// of the FP32 MFMA kernel; larger ranks are padded to the 32-column chunks of the BF16
+	// MFMA kernel (32 or 64), so every rank above 32 can run in BF16.
 	if (k <= 8)
 		return (k + 3u) & ~3u;
-	return (k + 15u) & ~15u;
+	if (k <= 16)
+		return 16u;
+	return (k + 31u) & ~31u;
 }
 
 static inline uint32_t default_ap(uint32_t kt)
@@ -1390,7 +1665,9 @@ extern "C" void nmfk_plan_half_target(uint32_t p, uint32_t q, uint32_t kt, int p
 				      nmfk_hplan *pl)
 {
 	if (path == NMFK_PATH_AUTO)
-		path = (kt % 16 == 0) ? NMFK_PATH_MFMA32 : NMFK_PATH_VALU;
+		path = kt >= 64 ? NMFK_PATH_MFMA_BF16 : (kt % 16 == 0) ? NMFK_PATH_MFMA32 : NMFK_PATH_VALU;
+	if (path == NMFK_PATH_MFMA_BF16 && kt % 32 != 0)
+		path = NMFK_PATH_MFMA32;
 	if (path != NMFK_PATH_VALU && kt % 16 != 0)
 		path = NMFK_PATH_VALU;
 	// rows a per workgroup = 64 * ap: VALU lanes own ap rows each; the MFMA kernel's NA
@@ -1401,7 +1678,7 @@ extern "C" void nmfk_plan_half_target(uint32_t p, uint32_t q, uint32_t kt, int p
 	const bool valu = path == NMFK_PATH_VALU;
 	const uint32_t ap = valu ? default_ap(kt) : (uint32_t)na;
 	const uint32_t nw = valu ? 4u : (uint32_t)(na * nb);
-	const uint32_t rows_per_step = valu ? 1u : 16u;           // rows of U per wave step
+	const uint32_t rows_per_step = valu ? 1u : path == NMFK_PATH_MFMA_BF16 ? 32u : 16u;   // rows of U per wave step
 	const uint32_t steppers = valu ? nw : (uint32_t)nb;        // wave groups splitting the rows
 	const uint32_t nab = (p + 64u * ap - 1u) / (64u * ap);
 	const uint32_t target =
@@ -1473,7 +1750,7 @@ extern "C" int nmfk_launch_partial(const float *U, size_t ldu, const float *A, c
 				   const float *colB, uint32_t ncolB, float *den,
 				   const nmfk_state *st, hipStream_t stream)
 {
-	if (ldu % 4 != 0)
+	if (ldu % 4 != 0 || pl->path == NMFK_PATH_MFMA_BF16)   // BF16: nmfk_launch_partial_bf16
 		return (int)hipErrorInvalidValue;
 	if (pl->path == NMFK_PATH_MFMA32) {
 		switch (kt) {
@@ -1500,18 +1777,91 @@ extern "C" int nmfk_launch_partial(const float *U, size_t ldu, const float *A, c
 	}
 }
 
-extern "C" int nmfk_launch_finish(const float *part, uint32_t S, float *A, uint32_t p, uint32_t k, uint32_t kt,
-				  const float *den, float *colA, int clamp, const nmfk_state *st,
-				  hipStream_t stream)
+extern "C" int nmfk_launch_finish_shadow(const float *part, uint32_t S, float *A, uint32_t p, uint32_t k,
+					 uint32_t kt, const float *den, float *colA, int clamp, const nmfk_state *st,
+					 uint16_t *Ab, uint16_t *Atb, uint32_t ldt, hipStream_t stream)
 {
 	const unsigned g = (p + NMFK_FIN_ROWS - 1) / NMFK_FIN_ROWS;
+	if (Ab != nullptr && (kt % 32 != 0 || ldt < p))
+		return (int)hipErrorInvalidValue;
+	const shadow_args sh = {(__bf16 *)Ab, (__bf16 *)Atb, (int)ldt};
 	switch (kt) {
 #define X(K)                                                                                           \
 	case K:                                                                                        \
 		hipLaunchKernelGGL((k_finish<K>), dim3(g), dim3(NMFK_WG), 0, stream, part, (int)S, A, (int)p, \
-				   (int)k, den, colA, clamp, st);                                       \
+				   (int)k, den, colA, clamp, st, sh);                                   \
 		return (int)hipGetLastError();
 		NMFK_KT_LIST(X)
+#undef X
+	default:
+		return (int)hipErrorInvalidValue;
+	}
+}
+
+extern "C" int nmfk_launch_finish(const float *part, uint32_t S, float *A, uint32_t p, uint32_t k, uint32_t kt,
+				  const float *den, float *colA, int clamp, const nmfk_state *st,
+				  hipStream_t stream)
+{
+	return nmfk_launch_finish_shadow(part, S, A, p, k, kt, den, colA, clamp, st, nullptr, nullptr, 0, stream);
+}
+
+extern "C" int nmfk_launch_shadow(const float *A, uint32_t p, uint32_t kt, uint16_t *Ab, uint16_t *Atb, uint32_t ldt,
+				  hipStream_t stream)
+{
+	const unsigned g = (p + NMFK_FIN_ROWS - 1) / NMFK_FIN_ROWS;
+	if (ldt < p)
+		return (int)hipErrorInvalidValue;
+	switch (kt) {
+#define X(K)                                                                                           \
+	case K:                                                                                        \
+		hipLaunchKernelGGL((k_shadow<K>), dim3(g), dim3(NMFK_WG), 0, stream, A, (int)p, (__bf16 *)Ab,  \
+				   (__bf16 *)Atb, (int)ldt);                                             \
+		return (int)hipGetLastError();
+		NMFK_KT_BF16_LIST(X)
+#undef X
+	default:
+		return (int)hipErrorInvalidValue;
+	}
+}
+
+template <int KT>
+static int launch_partial_bf16_t(const float *U, size_t ldu, const float *A, const uint16_t *Bb, const uint16_t *Btb,
+				 uint32_t ldbt, uint32_t p, uint32_t q, const nmfk_hplan *pl, float *part,
+				 const float *colB, uint32_t ncolB, float *den, const nmfk_state *st,
+				 hipStream_t stream)
+{
+	dim3 grid(pl->nab + 1, pl->S);   // + the denominator column
+	const int na = (int)pl->ap, nb = pl->ap ? (int)(pl->nw / pl->ap) : 0;
+	if (pl->bchunk % 32 != 0 || ldbt % 32 != 0 || ldbt < q || !mfma_shape_ok(na, nb))
+		return (int)hipErrorInvalidValue;
+	const half_args h = {U, ldu, A, nullptr, (int)p, (int)q, (int)pl->bchunk, (int)pl->nab, part, colB, (int)ncolB, den};
+	const bf16_args b = {(const __bf16 *)Bb, (const __bf16 *)Btb, (int)ldbt};
+#define NMFK_BF16_SHAPE(NA_, NB_)                                                                        \
+	if (na == NA_ && nb == NB_) {                                                                  \
+		hipLaunchKernelGGL((k_partial_bf16<KT, NA_, NB_>), grid, dim3(64 * NA_ * NB_), 0, stream, h, b, st); \
+		return (int)hipGetLastError();                                                         \
+	}
+	NMFK_BF16_SHAPE(1, 4)
+	NMFK_BF16_SHAPE(2, 2)
+	NMFK_BF16_SHAPE(4, 1)
+	NMFK_BF16_SHAPE(2, 4)
+	NMFK_BF16_SHAPE(4, 2)
+#undef NMFK_BF16_SHAPE
+	return (int)hipErrorInvalidValue;
+}
+
+extern "C" int nmfk_launch_partial_bf16(const float *U, size_t ldu, const float *A, const uint16_t *Bb,
+					const uint16_t *Btb, uint32_t ldbt, uint32_t p, uint32_t q, uint32_t kt,
+					const nmfk_hplan *pl, float *part, const float *colB, uint32_t ncolB,
+					float *den, const nmfk_state *st, hipStream_t stream)
+{
+	if (ldu % 4 != 0 || pl->path != NMFK_PATH_MFMA_BF16)
+		return (int)hipErrorInvalidValue;
+	switch (kt) {
+#define X(K)                                                                                          \
+	case K:                                                                                       \
+		return launch_partial_bf16_t<K>(U, ldu, A, Bb, Btb, ldbt, p, q, pl, part, colB, ncolB, den, st, stream);
+		NMFK_KT_BF16_LIST(X)
 #undef X
 	default:
 		return (int)hipErrorInvalidValue;

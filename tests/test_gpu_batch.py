@@ -113,7 +113,7 @@ def test_batch_autotune_and_tilings_keep_bits(nmfk, orc):
 
 def test_batch_rejects_bad_jobs(nmfk, orc):
     batch = nmfk.Batch(T, 2)
-    with nmfk.Job(300, 100, 40) as big:          # kt 48: not a batched width
+    with nmfk.Job(300, 100, 40) as big:          # kt 64: not a batched width
         big.init_random(1)
         with pytest.raises(nmfk.NmfkError):
             batch.add(big, 100, 4)

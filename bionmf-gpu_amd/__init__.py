@@ -86,6 +86,36 @@ class SweepOut(C.Structure):
                 ("seconds", C.c_double), ("tune_seconds", C.c_double), ("tuning", BatchTuning)]
 
 
+class Request(C.Structure):
+    _fields_ = [("n", C.c_uint32), ("m", C.c_uint32), ("k", C.c_uint32), ("restarts", C.c_uint32),
+                ("seed", C.c_uint64), ("arrival", C.c_double)]
+
+
+class RequestResult(C.Structure):
+    _fields_ = [("admitted", C.c_double), ("finished", C.c_double), ("latency", C.c_double),
+                ("iterations_max", C.c_uint32), ("iterations_sum", C.c_uint32),
+                ("converged", C.c_uint32), ("best_restart", C.c_uint32), ("kl_best", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class ServeConfig(C.Structure):
+    _fields_ = [("hbm_budget", C.c_size_t), ("capacity", C.c_uint32), ("nIters", C.c_uint32),
+                ("niter_test_conv", C.c_uint32), ("stop_threshold", C.c_uint32),
+                ("chunks_per_poll", C.c_uint32), ("mu", C.c_double), ("sigma", C.c_double)]
+
+
+class ServeStats(C.Structure):
+    _fields_ = [("seconds", C.c_double), ("requests", C.c_uint32), ("peak_jobs", C.c_uint32),
+                ("requests_per_s", C.c_double), ("factorizations_per_s", C.c_double),
+                ("p50", C.c_double), ("p99", C.c_double), ("mean_latency", C.c_double),
+                ("peak_bytes", C.c_size_t)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 _F32P = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
 _U32P = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
 
@@ -153,6 +183,10 @@ def load(path: os.PathLike | str | None = None):
     _sig(lib, "nmfk_sweep_default_config", None, [C.POINTER(SweepConfig)])
     _sig(lib, "nmfk_sweep_seed", u64, [u64, u32, u32])
     _sig(lib, "nmfk_sweep_run", i32, [vp, C.POINTER(SweepConfig), C.POINTER(SweepOut)])
+    _sig(lib, "nmfk_serve_default_config", None, [C.POINTER(ServeConfig)])
+    _sig(lib, "nmfk_request_bytes", sz, [u32, u32, u32, u32])
+    _sig(lib, "nmfk_serve_run", i32, [C.POINTER(ServeConfig), C.POINTER(Request), u32,
+                                      C.POINTER(RequestResult), C.POINTER(ServeStats)])
     _lib = lib
     return lib
 
@@ -454,3 +488,40 @@ def sweep(src: Job, ks, restarts: int, base_seed: int, nIters=2000, niter_test_c
                 facs.append((W, H))
         r["factors"] = facs
     return r
+
+
+def serve(requests, hbm_budget=200 << 30, capacity=1024, nIters=2000, niter_test_conv=10,
+          stop_threshold=40, mu=5.0, sigma=1.5, chunks_per_poll=1):
+    """nmfk_serve_run over [(n, m, k, restarts, seed, arrival_s), ...] (arrivals sorted).
+    Returns (per-request result dicts, stats dict)."""
+    lib = load()
+    cfg = ServeConfig()
+    lib.nmfk_serve_default_config(C.byref(cfg))
+    cfg.hbm_budget, cfg.capacity = hbm_budget, capacity
+    cfg.nIters, cfg.niter_test_conv, cfg.stop_threshold = nIters, niter_test_conv, stop_threshold
+    cfg.mu, cfg.sigma, cfg.chunks_per_poll = mu, sigma, chunks_per_poll
+    nr = len(requests)
+    req = (Request * max(nr, 1))()
+    for i, (n, m, k, r, seed, t) in enumerate(requests):
+        req[i] = Request(n, m, k, r, seed, t)
+    out = (RequestResult * max(nr, 1))()
+    st = ServeStats()
+    _check(lib.nmfk_serve_run(C.byref(cfg), req, nr, out, C.byref(st)), "serve")
+    return [out[i].as_dict() for i in range(nr)], st.as_dict()
+
+
+def request_stream(count, rate, seed, n_range=(1000, 60000), m_range=(20, 3000), k_range=(2, 32),
+                   restarts_range=(1, 10)):
+    """Synthetic request stream: Poisson arrivals at <rate> per second; n and m log-uniform
+    in their ranges, k uniform in {k_lo .. min(k_hi, m)}, restarts uniform; per-request
+    seeds.  Returns [(n, m, k, restarts, seed, arrival_s), ...]."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, count)) if rate > 0 else np.zeros(count)
+    out = []
+    for i in range(count):
+        n = int(round(np.exp(rng.uniform(np.log(n_range[0]), np.log(n_range[1])))))
+        m = int(round(np.exp(rng.uniform(np.log(m_range[0]), np.log(m_range[1])))))
+        k = int(rng.integers(k_range[0], min(k_range[1], m) + 1))
+        r = int(rng.integers(restarts_range[0], restarts_range[1] + 1))
+        out.append((n, m, k, r, int(rng.integers(0, 2**63)), float(t[i])))
+    return out

@@ -31,6 +31,7 @@ struct nmfk_batch {
 	nmfk_options *opt;
 	double *t_add;
 	uint32_t *slot_tiles;         /* tiles a slot holds in the current tables */
+	uint8_t *fresh;               /* added after the last launched chunk: its gathered state is not yet valid */
 	nmfk_bjob *h_desc, *d_desc;
 	nmfk_state *d_states, *h_states;
 	nmfk_tile *h_tiles, *d_tiles;
@@ -75,8 +76,9 @@ nmfk_batch *nmfk_batch_create(uint32_t niter_test_conv, uint32_t cap)
 	b->opt = (nmfk_options *)calloc(cap, sizeof(nmfk_options));
 	b->t_add = (double *)calloc(cap, sizeof(double));
 	b->slot_tiles = (uint32_t *)calloc(cap, sizeof(uint32_t));
+	b->fresh = (uint8_t *)calloc(cap, 1);
 	b->h_desc = (nmfk_bjob *)calloc(cap, sizeof(nmfk_bjob));
-	int e = (!b->job || !b->opt || !b->t_add || !b->slot_tiles || !b->h_desc) ? NMFK_ENOMEM : 0;
+	int e = (!b->job || !b->opt || !b->t_add || !b->slot_tiles || !b->fresh || !b->h_desc) ? NMFK_ENOMEM : 0;
 	if (e)
 		nmfk_set_error("nmfk_batch_create: out of host memory");
 	if (!e) e = hipck(hipMalloc((void **)&b->d_desc, cap * sizeof(nmfk_bjob)), "hipMalloc(batch jobs)");
@@ -114,7 +116,7 @@ void nmfk_batch_destroy(nmfk_batch *b)
 		hipEventDestroy(b->ev);
 	if (b->stream)
 		hipStreamDestroy(b->stream);
-	free(b->job), free(b->opt), free(b->t_add), free(b->slot_tiles), free(b->h_desc), free(b->h_tiles);
+	free(b->job), free(b->opt), free(b->t_add), free(b->slot_tiles), free(b->fresh), free(b->h_desc), free(b->h_tiles);
 	free(b);
 }
 
@@ -191,6 +193,7 @@ int nmfk_batch_add(nmfk_batch *b, nmfk_job *j, const nmfk_options *opt)
 		return -1;
 	}
 	b->job[s] = j;
+	b->fresh[s] = 1;
 	b->opt[s] = *opt;
 	b->t_add[s] = nmfk_now();
 	fill_desc(&b->h_desc[s], j);
@@ -404,6 +407,8 @@ int nmfk_batch_step(nmfk_batch *b, uint32_t chunks)
 		e = hipck(hipGraphLaunch(b->exec, b->stream), "batch chunk");
 	if (!e)
 		e = hipck(hipEventRecord(b->ev, b->stream), "batch event");
+	if (!e && chunks)
+		memset(b->fresh, 0, b->cap);   /* every added job is in the tables these chunks ran */
 	return e;
 }
 
@@ -427,7 +432,7 @@ int nmfk_batch_poll(nmfk_batch *b, nmfk_job **done, nmfk_result *res, uint32_t c
 	const double now = nmfk_now();
 	for (uint32_t s = 0; s < b->nslots && *ndone < cap; s++) {
 		nmfk_job *j = b->job[s];
-		if (!j)
+		if (!j || b->fresh[s])
 			continue;
 		const nmfk_state st = b->h_states[s];
 		if (!st.converged)

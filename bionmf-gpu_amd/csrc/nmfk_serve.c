@@ -1,0 +1,269 @@
+/*
+ * nmfk_serve.c -- serving loop: a stream of factorization requests on one MI355X.
+ *
+ * A request is one input matrix (generated on the device from its seed, as the synthetic
+ * benchmark inputs are) factorized at rank k from <restarts> seeded starting points.  The
+ * reference serves one request per process run (src/NMF_GPU.c:1034-1269); here requests
+ * arrive on a wall-clock schedule and are admitted in arrival order as soon as their
+ * arrival time has passed, their device memory (nmfk_job_bytes; restarts share V) fits in
+ * the HBM budget and the batch has free slots.  Every restart is a job of one shared
+ * nmfk_batch, so work of requests that arrived at different times overlaps on the device;
+ * a request retires when all its restarts have met the stop rule (or nIters), its memory
+ * is released and its latency (finish - arrival) recorded.
+ *
+ * The host loop enqueues the next chunk before doing admission work (matrix generation,
+ * job set-up), so the device keeps iterating while new requests are prepared.
+ */
+#include "nmfk_internal.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+void nmfk_serve_default_config(nmfk_serve_config *c)
+{
+	memset(c, 0, sizeof *c);
+	c->hbm_budget = (size_t)200 << 30;
+	c->capacity = 1024;
+	c->nIters = 2000;
+	c->niter_test_conv = 10;
+	c->stop_threshold = 40;
+	c->mu = 5.0;
+	c->sigma = 1.5;
+	c->chunks_per_poll = 1;
+}
+
+/* Device bytes of one request: its own job (V, Vt and buffers) plus restarts - 1 jobs that
+ * share V. */
+size_t nmfk_request_bytes(uint32_t n, uint32_t m, uint32_t k, uint32_t restarts)
+{
+	const size_t full = nmfk_job_bytes(n, m, k);
+	const size_t ldv = (m + 3u) & ~3u, ldvt = (n + 3u) & ~3u;
+	const size_t vbytes = 4 * ((size_t)n * ldv + (size_t)m * ldvt);
+	return full + (size_t)(restarts ? restarts - 1 : 0) * (full - vbytes);
+}
+
+typedef struct req_state {
+	nmfk_job **jobs;      /* restarts jobs; jobs[0] owns V */
+	uint32_t left;        /* restarts not finished */
+	size_t bytes;
+	int admitted;
+} req_state;
+
+static void free_request(req_state *r, uint32_t restarts)
+{
+	if (!r->jobs)
+		return;
+	for (uint32_t i = restarts; i-- > 0;)   /* shared jobs before the owner of V */
+		nmfk_job_destroy(r->jobs[i]);
+	free(r->jobs);
+	r->jobs = NULL;
+}
+
+static int cmp_double(const void *a, const void *b)
+{
+	const double x = *(const double *)a, y = *(const double *)b;
+	return x < y ? -1 : x > y;
+}
+
+static double quantile(double *sorted, uint32_t n, double q)
+{
+	if (n == 0)
+		return 0.0;
+	/* linear interpolation between closest ranks (numpy's default) */
+	const double pos = q * (double)(n - 1);
+	const uint32_t i = (uint32_t)pos;
+	const double f = pos - (double)i;
+	return i + 1 < n ? sorted[i] * (1.0 - f) + sorted[i + 1] * f : sorted[n - 1];
+}
+
+static void sleep_until(double t0, double t)
+{
+	double now = nmfk_now() - t0;
+	if (t <= now)
+		return;
+	const double d = t - now;
+	struct timespec ts;
+	ts.tv_sec = (time_t)d;
+	ts.tv_nsec = (long)((d - (double)ts.tv_sec) * 1e9);
+	nanosleep(&ts, NULL);
+}
+
+/* Creates the request's jobs: V from its seed, restart r from nmfk_sweep_seed(seed, k, r). */
+static int admit(const nmfk_serve_config *cfg, const nmfk_request *q, req_state *r, nmfk_batch *b)
+{
+	r->jobs = (nmfk_job **)calloc(q->restarts, sizeof(nmfk_job *));
+	if (!r->jobs) {
+		nmfk_set_error("out of host memory");
+		return NMFK_ENOMEM;
+	}
+	nmfk_options opt;
+	opt.k = q->k;
+	opt.nIters = cfg->nIters;
+	opt.niter_test_conv = cfg->niter_test_conv;
+	opt.stop_threshold = cfg->stop_threshold;
+	int e = 0;
+	r->jobs[0] = nmfk_job_create(q->n, q->m, q->k);
+	if (!r->jobs[0])
+		return NMFK_EINVAL;
+	e = nmfk_job_gen_V_lognormal(r->jobs[0], q->seed, cfg->mu, cfg->sigma);
+	for (uint32_t i = 1; i < q->restarts && !e; i++) {
+		r->jobs[i] = nmfk_job_create_shared(r->jobs[0], q->k);
+		if (!r->jobs[i])
+			e = NMFK_EINVAL;
+	}
+	for (uint32_t i = 0; i < q->restarts && !e; i++) {
+		e = nmfk_job_init_random(r->jobs[i], nmfk_sweep_seed(q->seed, q->k, i));
+		if (!e && nmfk_batch_add(b, r->jobs[i], &opt) < 0)
+			e = NMFK_EINVAL;
+	}
+	return e;
+}
+
+int nmfk_serve_run(const nmfk_serve_config *cfg, const nmfk_request *req, uint32_t nreq, nmfk_request_result *out,
+		   nmfk_serve_stats *stats)
+{
+	for (uint32_t i = 0; i < nreq; i++) {
+		const nmfk_request *q = &req[i];
+		if (q->restarts == 0 || q->k < 1 || q->k > NMFK_BATCH_KTMAX || (i && q->arrival < req[i - 1].arrival)) {
+			nmfk_set_error("nmfk_serve_run: request %u: need restarts >= 1, 1 <= k <= %d, arrivals in order", i,
+				       NMFK_BATCH_KTMAX);
+			return NMFK_EINVAL;
+		}
+		if (q->restarts > cfg->capacity || nmfk_request_bytes(q->n, q->m, q->k, q->restarts) > cfg->hbm_budget) {
+			nmfk_set_error("nmfk_serve_run: request %u can never fit (restarts > capacity or bytes > budget)", i);
+			return NMFK_EBUDGET;
+		}
+	}
+	nmfk_batch *b = nmfk_batch_create(cfg->niter_test_conv, cfg->capacity);
+	if (!b)
+		return NMFK_EINVAL;
+	req_state *rs = (req_state *)calloc(nreq ? nreq : 1, sizeof(req_state));
+	nmfk_job **done = (nmfk_job **)malloc(cfg->capacity * sizeof(nmfk_job *));
+	nmfk_result *dres = (nmfk_result *)malloc(cfg->capacity * sizeof(nmfk_result));
+	double *lat = (double *)malloc((nreq ? nreq : 1) * sizeof(double));
+	/* job -> request map, by slot-independent linear search over active requests' jobs */
+	if (!rs || !done || !dres || !lat) {
+		free(rs), free(done), free(dres), free(lat);
+		nmfk_batch_destroy(b);
+		nmfk_set_error("out of host memory");
+		return NMFK_ENOMEM;
+	}
+	for (uint32_t i = 0; i < nreq; i++) {
+		memset(&out[i], 0, sizeof out[i]);
+		out[i].kl_best = INFINITY;
+	}
+	size_t used = 0, peak = 0;
+	uint32_t active_jobs = 0, peak_jobs = 0, next = 0, finished = 0, first_active = 0;
+	int e = 0;
+	const double t0 = nmfk_now();
+	while (!e && finished < nreq) {
+		/* 1. keep the device busy */
+		if (nmfk_batch_active(b) > 0)
+			e = nmfk_batch_step(b, cfg->chunks_per_poll);
+		/* 2. admit what has arrived and fits, in arrival order */
+		while (!e && next < nreq) {
+			const nmfk_request *q = &req[next];
+			const double now = nmfk_now() - t0;
+			if (q->arrival > now)
+				break;
+			const size_t need = nmfk_request_bytes(q->n, q->m, q->k, q->restarts);
+			if (used + need > cfg->hbm_budget || active_jobs + q->restarts > cfg->capacity)
+				break;
+			e = admit(cfg, q, &rs[next], b);
+			if (e)
+				break;
+			rs[next].bytes = need;
+			rs[next].left = q->restarts;
+			rs[next].admitted = 1;
+			out[next].admitted = nmfk_now() - t0;
+			used += need;
+			active_jobs += q->restarts;
+			if (used > peak)
+				peak = used;
+			if (active_jobs > peak_jobs)
+				peak_jobs = active_jobs;
+			next++;
+		}
+		if (e)
+			break;
+		if (nmfk_batch_active(b) == 0) {
+			if (next < nreq)
+				sleep_until(t0, req[next].arrival);
+			continue;
+		}
+		/* 3. retire finished jobs and requests */
+		uint32_t nd = 0;
+		e = nmfk_batch_poll(b, done, dres, cfg->capacity, &nd);
+		for (uint32_t d = 0; d < nd && !e; d++) {
+			uint32_t qi = first_active, ri = 0;
+			int found = 0;
+			for (; qi < next && !found; qi++) {
+				if (!rs[qi].jobs)
+					continue;
+				for (ri = 0; ri < req[qi].restarts; ri++)
+					if (rs[qi].jobs[ri] == done[d]) {
+						found = 1;
+						break;
+					}
+				if (found)
+					break;
+			}
+			if (!found) {
+				nmfk_set_error("nmfk_serve_run: finished job not found");
+				e = NMFK_EINVAL;
+				break;
+			}
+			nmfk_request_result *o = &out[qi];
+			double kl = 0.0;
+			e = nmfk_job_dot_kl(done[d], NULL, NULL, &kl);
+			if (e)
+				break;
+			if (dres[d].iterations > o->iterations_max)
+				o->iterations_max = dres[d].iterations;
+			o->iterations_sum += dres[d].iterations;
+			o->converged += dres[d].converged;
+			if (kl < o->kl_best) {
+				o->kl_best = kl;
+				o->best_restart = ri;
+			}
+			if (--rs[qi].left == 0) {
+				o->finished = nmfk_now() - t0;
+				o->latency = o->finished - req[qi].arrival;
+				lat[finished++] = o->latency;
+				used -= rs[qi].bytes;
+				active_jobs -= req[qi].restarts;
+				free_request(&rs[qi], req[qi].restarts);
+				while (first_active < next && !rs[first_active].jobs)
+					first_active++;
+			}
+		}
+	}
+	const double t1 = nmfk_now() - t0;
+	for (uint32_t i = 0; i < nreq; i++)
+		free_request(&rs[i], req[i].restarts);
+	nmfk_batch_destroy(b);
+	if (!e && stats) {
+		qsort(lat, finished, sizeof(double), cmp_double);
+		uint64_t jobs = 0;
+		for (uint32_t i = 0; i < nreq; i++)
+			jobs += req[i].restarts;
+		memset(stats, 0, sizeof *stats);
+		stats->seconds = t1;
+		stats->requests = finished;
+		stats->requests_per_s = t1 > 0 ? finished / t1 : 0.0;
+		stats->factorizations_per_s = t1 > 0 ? (double)jobs / t1 : 0.0;
+		stats->p50 = quantile(lat, finished, 0.50);
+		stats->p99 = quantile(lat, finished, 0.99);
+		double s = 0.0;
+		for (uint32_t i = 0; i < finished; i++)
+			s += lat[i];
+		stats->mean_latency = finished ? s / finished : 0.0;
+		stats->peak_bytes = peak;
+		stats->peak_jobs = peak_jobs;
+	}
+	free(rs), free(done), free(dres), free(lat);
+	return e;
+}

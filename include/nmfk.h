@@ -197,6 +197,52 @@ void nmfk_sweep_default_config(nmfk_sweep_config *c);
 uint64_t nmfk_sweep_seed(uint64_t base, uint32_t k, uint32_t restart);
 int nmfk_sweep_run(nmfk_job *src, const nmfk_sweep_config *cfg, nmfk_sweep_out *out);
 
+/* ---------------------------------------------------------------------------------------
+ * Serving loop: requests arrive on a wall-clock schedule; each is an input matrix
+ * V ~ lognormal(mu, sigma^2) generated on the device from its seed and factorized at rank
+ * k from <restarts> starting points (restart r: nmfk_sweep_seed(seed, k, r)).  Requests
+ * are admitted in arrival order once their memory fits in the HBM budget and the batch
+ * has slots, advance together with every other admitted request, and retire when all
+ * their restarts have stopped.
+ * ------------------------------------------------------------------------------------- */
+typedef struct nmfk_request {
+	uint32_t n, m, k, restarts;   /* 1 <= k <= min(n, m, 32) */
+	uint64_t seed;
+	double arrival;               /* seconds after the start of nmfk_serve_run, non-decreasing */
+} nmfk_request;
+
+typedef struct nmfk_request_result {
+	double admitted, finished;    /* seconds after the start */
+	double latency;               /* finished - arrival */
+	uint32_t iterations_max;      /* over restarts */
+	uint32_t iterations_sum;
+	uint32_t converged;           /* restarts that met the stop rule */
+	uint32_t best_restart;        /* restart with the smallest KL divergence */
+	double kl_best;               /* its generalized KL divergence D(V || WH), FP64 */
+} nmfk_request_result;
+
+typedef struct nmfk_serve_config {
+	size_t hbm_budget;            /* bytes of device memory requests may hold at once */
+	uint32_t capacity;            /* jobs (restarts) advanced at once */
+	uint32_t nIters, niter_test_conv, stop_threshold;
+	uint32_t chunks_per_poll;
+	double mu, sigma;             /* V = exp(N(mu, sigma^2)) */
+} nmfk_serve_config;
+
+typedef struct nmfk_serve_stats {
+	double seconds;               /* start to last retirement */
+	uint32_t requests, peak_jobs;
+	double requests_per_s, factorizations_per_s;
+	double p50, p99, mean_latency;/* request latency, seconds */
+	size_t peak_bytes;            /* largest admitted total of nmfk_request_bytes */
+} nmfk_serve_stats;
+
+void nmfk_serve_default_config(nmfk_serve_config *cfg);
+/* Device bytes one request holds (its V once, plus every restart's buffers). */
+size_t nmfk_request_bytes(uint32_t n, uint32_t m, uint32_t k, uint32_t restarts);
+int nmfk_serve_run(const nmfk_serve_config *cfg, const nmfk_request *req, uint32_t nreq,
+		   nmfk_request_result *out, nmfk_serve_stats *stats);
+
 /* Raw device pointers and shape (for the drop-in layer and for measurements). */
 typedef struct nmfk_job_info {
 	uint32_t n, m, k, kt;

@@ -60,6 +60,83 @@ def cpu_baseline(sample_iters: int) -> dict:
                       f"(s/iter {s_iter:.4f}) extrapolated to {ITERS} iterations"}
 
 
+def bench_sweep(pkg) -> dict:
+    """Rank sweep: V 5000 x 200 planted rank 6, k = 2..10 x 20 restarts = 180 jobs, stop rule
+    (T=10, 40, 2000 iterations); batched (one launch per phase for all jobs, launch shape
+    from the autotuner) against the same jobs run one at a time with nmfk_job_run."""
+    n, m = 5000, 200
+    ks, R, base = list(range(2, 11)), 20, 4242
+    src = pkg.Job(n, m, 1)
+    src.gen_V_planted(777, 6, 0.3)
+    a = pkg.sweep(src, ks, R, base, plan_target=16, autotune=True, autotune_iters=10)
+    b = pkg.sweep(src, ks, R, base, plan_target=16, mode=pkg.SWEEP_ONE_AT_A_TIME)
+    c = pkg.sweep(src, ks, R, base, mode=pkg.SWEEP_ONE_AT_A_TIME)   # default single-job plans
+    src.close()
+    nj = len(ks) * R
+    same = all(x["iterations"] == y["iterations"] for x, y in zip(a["results"], b["results"]))
+    best_one = max(nj / b["seconds"], nj / c["seconds"])
+    return {"workload": f"V {n}x{m} planted rank 6, k=2..10 x {R} restarts = {nj} jobs, T=10, "
+                        "stop_threshold=40, nIters=2000",
+            "batched_jobs_per_s": nj / a["seconds"], "one_at_a_time_jobs_per_s": best_one,
+            "speedup": (nj / a["seconds"]) / best_one,
+            "one_at_a_time_same_plan_jobs_per_s": nj / b["seconds"],
+            "one_at_a_time_default_plan_jobs_per_s": nj / c["seconds"],
+            "identical_class_vectors": bool(np.array_equal(a["cls"], b["cls"])) and same,
+            "mean_iterations": float(np.mean([r["iterations"] for r in a["results"]])),
+            "converged_jobs": int(sum(r["converged"] for r in a["results"])),
+            "autotune_seconds": a["tune_seconds"], "tuning": a["tuning"]}
+
+
+def bench_dense_rank(pkg) -> dict:
+    """Dense rank: V 50000 x 2000 planted rank 48, k = 64, 300 iterations: BF16 contractions
+    (FP32 accumulation) against the FP32 matrix-core path on the same start."""
+    n, m, k, iters = 50000, 2000, 64, 300
+    job = pkg.Job(n, m, k)
+    job.gen_V_planted(4848, 48, 0.3)
+    out = {}
+    for path in (3, 2):
+        job.set_kernel_path(path)
+        job.init_random(9)
+        tm = job.time_iterations(10)
+        job.init_random(9)
+        t0 = time.perf_counter()
+        job.run(iters, 10, 10**9)
+        dt = time.perf_counter() - t0
+        dv, dvwh, kl = job.dot_kl()
+        out[path] = {"seconds": dt, "ms_per_iteration": tm["ms_per_iter"],
+                     "ms_partial": 0.5 * (tm["ms_partial_H"] + tm["ms_partial_W"]),
+                     "kl": kl, "W": job.get_W(), "H": job.get_H(), "cls": job.get_classification()}
+    job.close()
+    b, f = out[3], out[2]
+    rows = np.arange(0, n, 7)
+    WHb = b["W"][rows].astype(np.float64) @ b["H"].T.astype(np.float64)
+    WHf = f["W"][rows].astype(np.float64) @ f["H"].T.astype(np.float64)
+    part_bytes = 4.0 * (n * m + n * k + m * k)
+    return {"workload": f"V {n}x{m} planted rank 48, k={k}, {iters} iterations",
+            "bf16_factorizations_per_s": 1.0 / b["seconds"], "fp32_factorizations_per_s": 1.0 / f["seconds"],
+            "bf16_ms_per_iteration": b["ms_per_iteration"], "fp32_ms_per_iteration": f["ms_per_iteration"],
+            "bf16_partial_GBps": part_bytes / (b["ms_partial"] * 1e-3) / 1e9,
+            "fp32_partial_GBps": part_bytes / (f["ms_partial"] * 1e-3) / 1e9,
+            "bf16_vs_fp32": {"rel_frobenius_WH_rows_step7": float(np.linalg.norm(WHb - WHf) / np.linalg.norm(WHf)),
+                             "rel_kl_diff": abs(b["kl"] - f["kl"]) / f["kl"],
+                             "class_agreement": float(np.mean(b["cls"] == f["cls"]))}}
+
+
+def bench_served(pkg, count: int, rate: float) -> dict:
+    """Served stream: Poisson arrivals; n log-U[1000, 60000], m log-U[20, 3000], k U{2..32}
+    (at most m), restarts U{1..10}, V lognormal(5, 1.5^2); HBM budget 200 GB; stop rule as
+    the plain single factorization."""
+    reqs = pkg.request_stream(count, rate, 2024)
+    out, st = pkg.serve(reqs)
+    return {"workload": f"{count} requests, Poisson arrivals at {rate}/s, n log-U[1000,60000], "
+                        "m log-U[20,3000], k U{2..min(32,m)}, restarts U{1..10}, HBM budget 200 GB",
+            "requests_per_s": st["requests_per_s"], "factorizations_per_s": st["factorizations_per_s"],
+            "latency_s": {"p50": st["p50"], "p99": st["p99"], "mean": st["mean_latency"]},
+            "seconds": st["seconds"], "peak_jobs": st["peak_jobs"], "peak_GB": st["peak_bytes"] / 1e9,
+            "restarts": int(sum(r[3] for r in reqs)),
+            "mean_iterations_per_restart": float(sum(o["iterations_sum"] for o in out) / sum(r[3] for r in reqs))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -67,6 +144,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=30)
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the rank-sweep, dense-rank and served-stream measurements")
+    ap.add_argument("--served-requests", type=int, default=100)
+    ap.add_argument("--served-rate", type=float, default=5.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; the multi-GPU row-block "
@@ -118,6 +199,12 @@ def main():
     if tfile.exists():
         traffic = json.loads(tfile.read_text()).get("bytes_per_launch")
 
+    extra = {}
+    if not args.no_extra:
+        extra["sweep"] = bench_sweep(pkg)
+        extra["dense_rank"] = bench_dense_rank(pkg)
+        extra["served"] = bench_served(pkg, args.served_requests, args.served_rate)
+
     cpu = None
     if not args.no_cpu_baseline:
         try:
@@ -155,6 +242,7 @@ def main():
                      "frac_of_measured_read": achieved / read_gbs},
         "cpu_baseline": cpu,
     }
+    line.update(extra)
     job.close()
     print(json.dumps(line))
 

@@ -116,6 +116,13 @@ class ServeStats(C.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class Tags(C.Structure):
+    """nmfk_tags (include/nmfk_io.h)."""
+
+    _fields_ = [("name", C.c_char_p), ("headers", C.POINTER(C.c_char_p)), ("labels", C.POINTER(C.c_char_p)),
+                ("nheaders", C.c_uint32), ("nlabels", C.c_uint32)]
+
+
 _F32P = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
 _U32P = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
 
@@ -184,6 +191,11 @@ def load(path: os.PathLike | str | None = None):
     _sig(lib, "nmfk_sweep_seed", u64, [u64, u32, u32])
     _sig(lib, "nmfk_sweep_run", i32, [vp, C.POINTER(SweepConfig), C.POINTER(SweepOut)])
     _sig(lib, "nmfk_serve_default_config", None, [C.POINTER(ServeConfig)])
+    _sig(lib, "nmfk_tags_free", None, [C.POINTER(Tags)])
+    _sig(lib, "nmfk_matrix_load", i32, [C.c_char_p, i32, i32, i32, C.POINTER(C.POINTER(C.c_float)),
+                                        C.POINTER(u32), C.POINTER(u32), C.POINTER(Tags)])
+    _sig(lib, "nmfk_matrix_save", i32, [C.c_char_p, i32, C.POINTER(C.c_float), u32, u32, sz, i32,
+                                        C.POINTER(Tags)])
     _sig(lib, "nmfk_request_bytes", sz, [u32, u32, u32, u32])
     _sig(lib, "nmfk_serve_run", i32, [C.POINTER(ServeConfig), C.POINTER(Request), u32,
                                       C.POINTER(RequestResult), C.POINTER(ServeStats)])
@@ -525,3 +537,42 @@ def request_stream(count, rate, seed, n_range=(1000, 60000), m_range=(20, 3000),
         r = int(rng.integers(restarts_range[0], restarts_range[1] + 1))
         out.append((n, m, k, r, int(rng.integers(0, 2**63)), float(t[i])))
     return out
+
+
+FMT_TEXT, FMT_BINARY, FMT_NATIVE = 0, 1, 2
+
+
+def load_matrix(path, fmt=FMT_TEXT, numeric_hdrs=False, numeric_lbls=False):
+    """nmfk_matrix_load: returns (matrix float32 rows x cols, name, headers, labels)."""
+    lib = load()
+    A = C.POINTER(C.c_float)()
+    r, c_ = C.c_uint32(), C.c_uint32()
+    t = Tags()
+    _check(lib.nmfk_matrix_load(str(path).encode(), fmt, int(numeric_hdrs), int(numeric_lbls), C.byref(A),
+                                C.byref(r), C.byref(c_), C.byref(t)), "load_matrix")
+    M = np.ctypeslib.as_array(A, shape=(r.value, c_.value)).copy()
+    C.CDLL(None).free(C.cast(A, C.c_void_p))
+    name = t.name.decode() if t.name else None
+    hd = [t.headers[i].decode() for i in range(t.nheaders)] if t.headers else None
+    lb = [t.labels[i].decode() for i in range(t.nlabels)] if t.labels else None
+    lib.nmfk_tags_free(C.byref(t))
+    return M, name, hd, lb
+
+
+def save_matrix(path, A, fmt=FMT_TEXT, transpose=False, name=None, headers=None, labels=None):
+    """nmfk_matrix_save; tags describe the matrix as written (after the optional transpose)."""
+    lib = load()
+    A = np.ascontiguousarray(A, np.float32)
+    keep = []
+
+    def arr(lst):
+        if lst is None:
+            return None
+        a = (C.c_char_p * len(lst))(*[x.encode() for x in lst])
+        keep.append(a)
+        return C.cast(a, C.POINTER(C.c_char_p))
+    t = Tags(name.encode() if name is not None else None, arr(headers), arr(labels),
+             len(headers) if headers else 0, len(labels) if labels else 0)
+    r, c_ = A.shape
+    _check(lib.nmfk_matrix_save(str(path).encode(), fmt, A.ctypes.data_as(C.POINTER(C.c_float)), r, c_, c_,
+                                int(bool(transpose)), C.byref(t)), "save_matrix")

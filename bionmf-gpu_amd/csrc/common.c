@@ -64,16 +64,16 @@ DEFINE_PRINTER(append_printed_error, stderr, false, 0)
 DEFINE_PRINTER(print_errnum, stderr, true, 1)
 DEFINE_PRINTER(append_printed_errnum, stderr, false, 1)
 
-void flush_output(bool all_processes)
+int flush_output(bool permanently)
 {
-	(void)all_processes;
-	fflush(stdout);
-	fflush(stderr);
+	(void)permanently;
+	const int a = fflush(stdout), b = fflush(stderr);
+	return (a || b) ? EXIT_FAILURE : EXIT_SUCCESS;
 }
 
 /* ---------------------------------------------------------------- help */
 
-void help_file_formats(void)
+int help_file_formats(void)
 {
 	print_message(false,
 		      "Input/output file formats:\n"
@@ -83,9 +83,10 @@ void help_file_formats(void)
 		      "  Non-native binary (-b 0 / -e 0): file signature, uint32 dimensions, little-endian\n"
 		      "    FP64 data.\n"
 		      "  Native binary (-b 1 / -e 1): native-endian dimensions and 'real' data, no signature.\n");
+	return EXIT_SUCCESS;
 }
 
-void help_nmf(void)
+int help_nmf(void)
 {
 	print_message(false,
 		      "NMF options:\n"
@@ -95,16 +96,17 @@ void help_nmf(void)
 		      "  -t <tests>     stop after this many unchanged tests, plus one (default %" PRI_IDX ")\n"
 		      "  -z <device>    GPU device (default %" PRI_IDX ")\n",
 		      DEFAULT_K, DEFAULT_NITERS, DEFAULT_NITER_CONV, DEFAULT_STOP_THRESHOLD, DEFAULT_GPU_DEVICE);
+	return EXIT_SUCCESS;
 }
 
-void print_nmf_gpu_help(char const *execname)
+int print_nmf_gpu_help(char const *execname)
 {
 	print_message(false,
 		      "Usage: %s <filename> [-b <native>] [-c] [-r] [-e <native>] [-k <rank>] [-i <iters>]\n"
 		      "          [-j <iters>] [-t <tests>] [-z <device>] [-h]\n\n",
 		      execname ? execname : "NMF_GPU");
 	help_file_formats();
-	help_nmf();
+	return help_nmf();
 }
 
 /* ---------------------------------------------------------------- limits */
@@ -139,21 +141,21 @@ index_t get_padding(index_t dim)
 	return r ? dim + (memory_alignment - r) : dim;
 }
 
-index_t next_power_2(index_t x)
+size_t next_power_2(size_t x)
 {
 	if (x <= 1)
 		return 1;
-	index_t p = 1;
-	while (p < x && p <= IDX_MAX / 2)
+	size_t p = 1;
+	while (p < x && p <= SIZE_MAX / 2)
 		p <<= 1;
 	return p;
 }
 
-index_t prev_power_2(index_t x)
+size_t prev_power_2(size_t x)
 {
 	if (x <= 1)
 		return 1;
-	index_t p = 1;
+	size_t p = 1;
 	while (p <= x / 2)
 		p <<= 1;
 	return p;

@@ -70,12 +70,12 @@ int print_error(bool all_processes, char const *fmt, ...);
 int append_printed_error(bool all_processes, char const *fmt, ...);
 int print_errnum(bool all_processes, int errnum, char const *fmt, ...);
 int append_printed_errnum(bool all_processes, int errnum, char const *fmt, ...);
-void flush_output(bool all_processes);
+int flush_output(bool permanently);
 
 /* Help texts (src/common.c:567-800). */
-void help_file_formats(void);
-void help_nmf(void);
-void print_nmf_gpu_help(char const *execname);
+int help_file_formats(void);
+int help_nmf(void);
+int print_nmf_gpu_help(char const *execname);
 
 /* Matrix limits (src/common.c:725-834): alignment (0 = default, 16 items), largest
  * dimension (0 = none) and largest number of items (0 = none).  Return EXIT_SUCCESS. */
@@ -88,8 +88,8 @@ index_t get_padding(index_t dim);
 int check_arguments(int argc, char const *const *argv, bool *help_requested,
 		    struct input_arguments *arguments);
 
-index_t next_power_2(index_t x);
-index_t prev_power_2(index_t x);
+size_t next_power_2(size_t x);
+size_t prev_power_2(size_t x);
 
 /* Number of pairs i<j whose "same class" relation differs between the two vectors
  * (src/common.c:1195-1209). */

@@ -225,3 +225,66 @@ def ref_common():
     L.get_difference.restype = C.c_size_t
     L.get_difference.argtypes = [_U32, _U32, C.c_uint32]
     return L
+
+
+REF_MATRIX_IO = pathlib.Path(__file__).resolve().parent / "_ref" / "libref_matrix_io.so"
+
+
+class _RefTag(C.Structure):
+    _fields_ = [("tokens", C.c_char_p), ("ptokens", C.POINTER(C.c_char_p))]
+
+
+class _RefTags(C.Structure):
+    _fields_ = [("name", C.c_char_p), ("headers", _RefTag), ("labels", _RefTag)]
+
+
+def ref_matrix_io():
+    """The reference's src/matrix_io/*.c + src/common.c compiled unmodified (None when not
+    built here).  Returns (load, save): load(path, fmt, numeric_hdrs, numeric_lbls) ->
+    (matrix float32 rows x cols, name, headers, labels) or None on failure; save(path, fmt,
+    A, transpose, name, headers, labels) -> rc."""
+    if not REF_MATRIX_IO.exists():
+        return None
+    L = C.CDLL(str(REF_MATRIX_IO))
+    L.set_default_matrix_limits()
+    u32 = C.c_uint32
+    L.matrix_load.argtypes = [C.c_char_p, C.c_bool, C.c_bool, C.c_int, C.POINTER(C.POINTER(C.c_float)),
+                              C.POINTER(u32), C.POINTER(u32), C.POINTER(u32), C.POINTER(_RefTags)]
+    L.matrix_save.argtypes = [C.c_char_p, C.c_int, C.POINTER(C.c_float), u32, u32, u32, C.c_bool,
+                              C.POINTER(_RefTags), C.c_bool]
+
+    def toks(tag, n):
+        if not tag.ptokens:
+            return None
+        return [tag.ptokens[i].decode() if tag.ptokens[i] else "" for i in range(n)]
+
+    def load(path, fmt, numeric_hdrs=False, numeric_lbls=False):
+        A = C.POINTER(C.c_float)()
+        r, c_, pitch = u32(), u32(), u32()
+        mt = _RefTags()
+        rc = L.matrix_load(str(path).encode(), numeric_hdrs, numeric_lbls, fmt, C.byref(A), C.byref(r),
+                           C.byref(c_), C.byref(pitch), C.byref(mt))
+        if rc != 0:
+            return None
+        M = np.ctypeslib.as_array(A, shape=(r.value, pitch.value))[:, :c_.value].copy()
+        name = mt.name.decode() if mt.name else None
+        return M, name, toks(mt.headers, c_.value), toks(mt.labels, r.value)
+
+    def save(path, fmt, A, transpose=False, name=None, headers=None, labels=None):
+        A = np.ascontiguousarray(A, np.float32)
+        keep = []
+
+        def mk(lst):
+            if lst is None:
+                return _RefTag()
+            arr = (C.c_char_p * len(lst))(*[x.encode() for x in lst])
+            keep.append(arr)
+            blob = C.create_string_buffer(b"\0".join(x.encode() for x in lst) + b"\0")
+            keep.append(blob)
+            return _RefTag(C.cast(blob, C.c_char_p), C.cast(arr, C.POINTER(C.c_char_p)))
+        nm = C.c_char_p(name.encode()) if name is not None else None
+        mt = _RefTags(nm, mk(headers), mk(labels))
+        r, c_ = A.shape
+        return L.matrix_save(str(path).encode(), fmt, A.ctypes.data_as(C.POINTER(C.c_float)), r, c_, c_,
+                             transpose, C.byref(mt), False)
+    return load, save

@@ -1,0 +1,79 @@
+"""The command-line driver bionmf-gpu_amd/NMF_GPU (the reference's bin/NMF_GPU): options,
+messages and output files, and its results equal the job API's on the same seed."""
+from __future__ import annotations
+
+import os
+import pathlib
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+pytestmark = pytest.mark.gpu
+
+DRIVER = ROOT / "bionmf-gpu_amd" / "NMF_GPU"
+
+
+def run_driver(args, seed=5):
+    if not DRIVER.exists():
+        pytest.fail("bionmf-gpu_amd/NMF_GPU is not built")
+    env = dict(os.environ, NMFGPU_SEED=str(seed))
+    return subprocess.run([str(DRIVER)] + [str(a) for a in args], capture_output=True, text=True, env=env,
+                          timeout=120)
+
+
+def test_driver_text_io_matches_job_api(nmfk, orc, tmp_path):
+    n, m, k = 300, 40, 3
+    V = orc.gen_lognormal(n, m, 17, 3.0, 1.0)
+    hd = [f"sample_{j}" for j in range(m)]
+    lb = [f"gene_{i}" for i in range(n)]
+    f = tmp_path / "input.txt"
+    nmfk.save_matrix(f, V, nmfk.FMT_TEXT, name="Expression", headers=hd, labels=lb)
+    Vf = nmfk.load_matrix(f)[0]                 # the values the driver reads (%g text)
+    r = run_driver([f, "-k", k, "-i", 500, "-j", 10, "-t", 20])
+    assert r.returncode == 0, r.stderr
+    assert "Starting NMF( K=3 )..." in r.stdout and "Distance between V and W*H:" in r.stdout
+    W, nm, whd, wlb = nmfk.load_matrix(tmp_path / "input.txt_W.txt")
+    H, hnm, hhd, hlb = nmfk.load_matrix(tmp_path / "input.txt_H.txt")
+    assert W.shape == (n, k) and H.shape == (k, m)
+    assert nm == hnm == "Expression" and wlb == lb and whd == [f"Factor_{i}" for i in range(k)]
+    assert hhd == hd and hlb == [f"Factor_{i}" for i in range(k)]
+    cls_lines = (tmp_path / "input.txt_classes.txt").read_text().splitlines()
+    cls = np.array([int(x.split("\t")[1]) for x in cls_lines], np.uint32)
+    with nmfk.Job(n, m, k) as j:
+        j.set_V(Vf)
+        j.init_random(5)
+        res = j.run(500, 10, 20)
+        W_ref, H_ref = j.get_W(), j.get_H()
+        assert np.array_equal(cls, j.get_classification())
+    # text output carries 6 significant digits (%g)
+    assert np.allclose(W, W_ref, rtol=1e-5, atol=0) and np.allclose(H, H_ref.T, rtol=1e-5, atol=0)
+    word = "converged in" if res["converged"] else "iterations performed"
+    assert word in r.stdout and str(res["iterations"]) in r.stdout
+
+
+def test_driver_binary_io_and_errors(nmfk, orc, tmp_path):
+    n, m, k = 200, 30, 4
+    V = orc.gen_lognormal(n, m, 3, 2.0, 1.0)
+    f = tmp_path / "in.dat"
+    nmfk.save_matrix(f, V, nmfk.FMT_BINARY)
+    r = run_driver([f, "-b", 0, "-e", 0, "-k", k, "-i", 200])
+    assert r.returncode == 0, r.stderr
+    W = nmfk.load_matrix(tmp_path / "in.dat_W.dat", nmfk.FMT_BINARY)[0]
+    H = nmfk.load_matrix(tmp_path / "in.dat_H.dat", nmfk.FMT_BINARY)[0]
+    with nmfk.Job(n, m, k) as j:
+        j.set_V(V)
+        j.init_random(5)
+        j.run(200, 10, 40)
+        assert np.array_equal(W, j.get_W()) and np.array_equal(H, j.get_H().T)   # exact: FP32 -> FP64
+    # native output keeps H as stored (M x K)
+    r = run_driver([f, "-b", 0, "-e", 1, "-k", k, "-i", 200])
+    assert r.returncode == 0, r.stderr
+    assert nmfk.load_matrix(tmp_path / "in.dat_H.native.dat", nmfk.FMT_NATIVE)[0].shape == (m, k)
+    # errors: rank above min(N, M), missing file
+    r = run_driver([f, "-b", 0, "-k", m + 1])
+    assert r.returncode != 0 and "invalid factorization rank" in r.stderr
+    r = run_driver([tmp_path / "missing.txt"])
+    assert r.returncode != 0

@@ -337,13 +337,33 @@ __device__ __forceinline__ void partial_valu_body(int bx, int by, const half_arg
 	}
 }
 
+// Single-job partial grids are 1-D: the (nab + 1) x S tiles, rounded up to a multiple of
+// the 8 XCDs.  The dispatcher places workgroup L on XCD L % 8; tile T = (L % 8) * per + L / 8
+// gives every XCD a contiguous run of tiles in by-major order, so the workgroups that share
+// a split of B (same by) mostly share one L2 instead of fetching it on every XCD.  Returns
+// false for the padding workgroups.
+__device__ __forceinline__ bool xcd_tile(int gx, int total, int &bx, int &by)
+{
+	const int per = (total + 7) >> 3;
+	const int L = blockIdx.x;
+	const int T = (L & 7) * per + (L >> 3);
+	if (T >= total)
+		return false;
+	by = T / gx;
+	bx = T - by * gx;
+	return true;
+}
+
 template <int KT, int AP, int NW>
-__global__ __launch_bounds__(64 * NW) void k_partial(half_args h, const nmfk_state *__restrict__ st)
+__global__ __launch_bounds__(64 * NW) void k_partial(half_args h, const nmfk_state *__restrict__ st, int S)
 {
 	__shared__ float smem[smem_valu(KT, AP, NW)];
 	if (st != nullptr && st->converged)
 		return;
-	partial_valu_body<KT, AP, NW>(blockIdx.x, blockIdx.y, h, smem);
+	int bx, by;
+	if (!xcd_tile(h.nab + 1, (h.nab + 1) * S, bx, by))
+		return;
+	partial_valu_body<KT, AP, NW>(bx, by, h, smem);
 }
 
 // --------------------------------------------------------------------------------------
@@ -642,12 +662,15 @@ __device__ __forceinline__ void partial_mfma_body(int bx, int by, const half_arg
 }
 
 template <int KT, int NA, int NB>
-__global__ __launch_bounds__(64 * NA * NB) void k_partial_mfma(half_args h, const nmfk_state *__restrict__ st)
+__global__ __launch_bounds__(64 * NA * NB) void k_partial_mfma(half_args h, const nmfk_state *__restrict__ st, int S)
 {
 	__shared__ float smem[smem_mfma(KT, NA, NB)];
 	if (st != nullptr && st->converged)
 		return;
-	partial_mfma_body<KT, NA, NB>(blockIdx.x, blockIdx.y, h, smem);
+	int bx, by;
+	if (!xcd_tile(h.nab + 1, (h.nab + 1) * S, bx, by))
+		return;
+	partial_mfma_body<KT, NA, NB>(bx, by, h, smem);
 }
 
 // --------------------------------------------------------------------------------------
@@ -873,12 +896,16 @@ __device__ __forceinline__ void partial_bf16_body(int bx, int by, const half_arg
 }
 
 template <int KT, int NA, int NB>
-__global__ __launch_bounds__(64 * NA * NB) void k_partial_bf16(half_args h, bf16_args b, const nmfk_state *__restrict__ st)
+__global__ __launch_bounds__(64 * NA * NB) void k_partial_bf16(half_args h, bf16_args b, const nmfk_state *__restrict__ st,
+							     int S)
 {
 	__shared__ float smem[smem_bf16(KT, NA, NB)];
 	if (st != nullptr && st->converged)
 		return;
-	partial_bf16_body<KT, NA, NB>(blockIdx.x, blockIdx.y, h, b, smem);
+	int bx, by;
+	if (!xcd_tile(h.nab + 1, (h.nab + 1) * S, bx, by))
+		return;
+	partial_bf16_body<KT, NA, NB>(bx, by, h, b, smem);
 }
 
 // A[a][c] = (A[a][c] * sum_s part[s][a][c]) / den[c], optional clamp, then the column sums
@@ -1708,16 +1735,24 @@ extern "C" void nmfk_plan_half(uint32_t p, uint32_t q, uint32_t kt, nmfk_hplan *
 	nmfk_plan_half_path(p, q, kt, NMFK_PATH_AUTO, pl);
 }
 
+// (nab + 1) x S tiles (+ the denominator column) as a 1-D grid padded to a multiple of 8
+// (see xcd_tile).
+static inline dim3 xcd_grid(const nmfk_hplan *pl)
+{
+	const unsigned total = (pl->nab + 1) * pl->S;
+	return dim3((total + 7u) & ~7u);
+}
+
 template <int KT, int AP>
 static int launch_partial_t(const float *U, size_t ldu, const float *A, const float *B, uint32_t p,
 			    uint32_t q, const nmfk_hplan *pl, float *part, const float *colB, uint32_t ncolB,
 			    float *den, const nmfk_state *st, hipStream_t stream)
 {
-	dim3 grid(pl->nab + 1, pl->S);   // + the denominator column
+	const dim3 grid = xcd_grid(pl);
 	if (pl->nw != 4)
 		return (int)hipErrorInvalidValue;
 	const half_args h = {U, ldu, A, B, (int)p, (int)q, (int)pl->bchunk, (int)pl->nab, part, colB, (int)ncolB, den};
-	hipLaunchKernelGGL((k_partial<KT, AP, 4>), grid, dim3(256), 0, stream, h, st);
+	hipLaunchKernelGGL((k_partial<KT, AP, 4>), grid, dim3(256), 0, stream, h, st, (int)pl->S);
 	return (int)hipGetLastError();
 }
 
@@ -1726,14 +1761,14 @@ static int launch_partial_mfma_t(const float *U, size_t ldu, const float *A, con
 				 uint32_t q, const nmfk_hplan *pl, float *part, const float *colB, uint32_t ncolB,
 				 float *den, const nmfk_state *st, hipStream_t stream)
 {
-	dim3 grid(pl->nab + 1, pl->S);   // + the denominator column
+	const dim3 grid = xcd_grid(pl);
 	const int na = (int)pl->ap, nb = pl->ap ? (int)(pl->nw / pl->ap) : 0;
 	if (pl->bchunk % 16 != 0 || !mfma_shape_ok(na, nb) || (size_t)q * ldu * 4 > 0x7fffffffull)
 		return (int)hipErrorInvalidValue;
 	const half_args h = {U, ldu, A, B, (int)p, (int)q, (int)pl->bchunk, (int)pl->nab, part, colB, (int)ncolB, den};
 #define NMFK_MFMA_SHAPE(NA_, NB_)                                                                        \
 	if (na == NA_ && nb == NB_) {                                                                  \
-		hipLaunchKernelGGL((k_partial_mfma<KT, NA_, NB_>), grid, dim3(64 * NA_ * NB_), 0, stream, h, st); \
+		hipLaunchKernelGGL((k_partial_mfma<KT, NA_, NB_>), grid, dim3(64 * NA_ * NB_), 0, stream, h, st, (int)pl->S); \
 		return (int)hipGetLastError();                                                         \
 	}
 	NMFK_MFMA_SHAPE(1, 4)
@@ -1830,7 +1865,7 @@ static int launch_partial_bf16_t(const float *U, size_t ldu, const float *A, con
 				 const float *colB, uint32_t ncolB, float *den, const nmfk_state *st,
 				 hipStream_t stream)
 {
-	dim3 grid(pl->nab + 1, pl->S);   // + the denominator column
+	const dim3 grid = xcd_grid(pl);
 	const int na = (int)pl->ap, nb = pl->ap ? (int)(pl->nw / pl->ap) : 0;
 	if (pl->bchunk % 32 != 0 || ldbt % 32 != 0 || ldbt < q || !mfma_shape_ok(na, nb))
 		return (int)hipErrorInvalidValue;
@@ -1838,7 +1873,7 @@ static int launch_partial_bf16_t(const float *U, size_t ldu, const float *A, con
 	const bf16_args b = {(const __bf16 *)Bb, (const __bf16 *)Btb, (int)ldbt};
 #define NMFK_BF16_SHAPE(NA_, NB_)                                                                        \
 	if (na == NA_ && nb == NB_) {                                                                  \
-		hipLaunchKernelGGL((k_partial_bf16<KT, NA_, NB_>), grid, dim3(64 * NA_ * NB_), 0, stream, h, b, st); \
+		hipLaunchKernelGGL((k_partial_bf16<KT, NA_, NB_>), grid, dim3(64 * NA_ * NB_), 0, stream, h, b, st, (int)pl->S); \
 		return (int)hipGetLastError();                                                         \
 	}
 	NMFK_BF16_SHAPE(1, 4)

@@ -835,18 +835,72 @@ __device__ __forceinline__ void partial_bf16_body(int bx, int by, const half_arg
 	const int mine = idle ? 0 : (nfull > wb ? (nfull - wb + NB - 1) / NB : 0);
 	auto blk_row = [&](int i) { return b_begin + 32 * (wb + NB * min(i, mine - 1)); };
 	if (mine > 0) {
-		// Two-slot ring: the next block's loads are in flight while this one is computed; a
-		// refill past the end re-reads this wave's last block and is never consumed.
-		blk ring[2];
-		load(blk_row(0), ring[0], true);
-		load(blk_row(1), ring[1], true);
-		for (int i = 0; i < mine; i += 2) {
-			compute(ring[0], 0, true);
-			load(blk_row(i + 2), ring[0], true);
-			if (i + 1 < mine) {
-				compute(ring[1], 0, true);
-				load(blk_row(i + 3), ring[1], true);
+		// U (the HBM stream) has a two-block ring.  The two B operands (L2-resident) are single
+		// buffers refilled as soon as their product has consumed them: GEMM1's right after
+		// GEMM1, GEMM2's right after GEMM2, so each refill overlaps the rest of the block.
+		// Refills past the end re-read this wave's last block and are never consumed.
+		float4 ur[2][2][4];
+		bf16x8 b1[2][CH], b2[H];
+		auto load_u = [&](int bb, float4 (&u)[2][4]) {
+#pragma unroll
+			for (int hb = 0; hb < 2; hb++)
+#pragma unroll
+				for (int r = 0; r < 4; r++)
+					u[hb][r] = *reinterpret_cast<const float4 *>(U + (size_t)(bb + 16 * hb + 4 * g + r) * ldu + ucol);
+		};
+		auto load_b1 = [&](int bb) {
+#pragma unroll
+			for (int hb = 0; hb < 2; hb++)
+#pragma unroll
+				for (int ch = 0; ch < CH; ch++)
+					b1[hb][ch] = *reinterpret_cast<const bf16x8 *>(Bb + (size_t)(bb + 16 * hb + j) * KT + 32 * ch + 8 * g);
+		};
+		auto load_b2 = [&](int bb) {
+#pragma unroll
+			for (int h = 0; h < H; h++) {
+				const __bf16 *col = Btb + (size_t)(16 * h + j) * ldbt + bb + 4 * g;
+				const bf16x4 lo = *reinterpret_cast<const bf16x4 *>(col);
+				const bf16x4 hi = *reinterpret_cast<const bf16x4 *>(col + 16);
+				b2[h] = bf16x8{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
 			}
+		};
+		auto step = [&](float4 (&u)[2][4], int next_b, int next_u) {
+			bf16x8 rt[4];
+#pragma unroll
+			for (int hb = 0; hb < 2; hb++) {
+				floatx4 sv[4];
+#pragma unroll
+				for (int t = 0; t < 4; t++) {
+					sv[t] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+					for (int ch = 0; ch < CH; ch++)
+						sv[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b1[hb][ch], aop[t][ch], sv[t], 0, 0, 0);
+				}
+#pragma unroll
+				for (int r = 0; r < 4; r++) {
+					const float uv[4] = {u[hb][r].x, u[hb][r].y, u[hb][r].z, u[hb][r].w};
+#pragma unroll
+					for (int t = 0; t < 4; t++)
+						rt[t][4 * hb + r] = (__bf16)(uv[t] * __builtin_amdgcn_rcpf(sv[t][r]));
+				}
+			}
+			load_b1(next_b);
+			load_u(next_u, u);
+#pragma unroll
+			for (int t = 0; t < 4; t++)
+#pragma unroll
+				for (int h = 0; h < H; h++)
+					acc[t][h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(rt[t], b2[h], acc[t][h], 0, 0, 0);
+			load_b2(next_b);
+		};
+		load_u(blk_row(0), ur[0]);
+		load_u(blk_row(1), ur[1]);
+		load_b1(blk_row(0));
+		load_b2(blk_row(0));
+		for (int i = 0; i < mine; i += 2) {
+			step(ur[0], blk_row(i + 1), blk_row(i + 2));
+			if (i + 1 < mine)
+				step(ur[1], blk_row(i + 2), blk_row(i + 3));
 		}
 	}
 	if (!idle && (b_end - b_begin) % 32 != 0 && wb == nfull % NB) {

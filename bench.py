@@ -68,8 +68,8 @@ def bench_sweep(pkg) -> dict:
     ks, R, base = list(range(2, 11)), 20, 4242
     src = pkg.Job(n, m, 1)
     src.gen_V_planted(777, 6, 0.3)
-    a = pkg.sweep(src, ks, R, base, plan_target=16, autotune=True, autotune_iters=10)
-    b = pkg.sweep(src, ks, R, base, plan_target=16, mode=pkg.SWEEP_ONE_AT_A_TIME)
+    a = pkg.sweep(src, ks, R, base, autotune=True, autotune_iters=10)   # plan and launch shape autotuned
+    b = pkg.sweep(src, ks, R, base, plan_target=a["plan_target"], mode=pkg.SWEEP_ONE_AT_A_TIME)
     c = pkg.sweep(src, ks, R, base, mode=pkg.SWEEP_ONE_AT_A_TIME)   # default single-job plans
     src.close()
     nj = len(ks) * R
@@ -84,7 +84,7 @@ def bench_sweep(pkg) -> dict:
             "identical_class_vectors": bool(np.array_equal(a["cls"], b["cls"])) and same,
             "mean_iterations": float(np.mean([r["iterations"] for r in a["results"]])),
             "converged_jobs": int(sum(r["converged"] for r in a["results"])),
-            "autotune_seconds": a["tune_seconds"], "tuning": a["tuning"]}
+            "autotune_seconds": a["tune_seconds"], "tuning": a["tuning"], "plan_target": a["plan_target"]}
 
 
 def bench_dense_rank(pkg) -> dict:

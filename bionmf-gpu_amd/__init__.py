@@ -83,7 +83,8 @@ class SweepConfig(C.Structure):
 class SweepOut(C.Structure):
     _fields_ = [("res", C.POINTER(Result)), ("cls", C.POINTER(C.c_uint32)),
                 ("consensus", C.POINTER(C.c_int32)), ("W_H_out", C.POINTER(C.c_float)),
-                ("seconds", C.c_double), ("tune_seconds", C.c_double), ("tuning", BatchTuning)]
+                ("seconds", C.c_double), ("tune_seconds", C.c_double), ("tuning", BatchTuning),
+                ("plan_target", C.c_uint32)]
 
 
 class Request(C.Structure):
@@ -103,7 +104,8 @@ class RequestResult(C.Structure):
 class ServeConfig(C.Structure):
     _fields_ = [("hbm_budget", C.c_size_t), ("capacity", C.c_uint32), ("nIters", C.c_uint32),
                 ("niter_test_conv", C.c_uint32), ("stop_threshold", C.c_uint32),
-                ("chunks_per_poll", C.c_uint32), ("mu", C.c_double), ("sigma", C.c_double)]
+                ("chunks_per_poll", C.c_uint32), ("plan_target", C.c_uint32), ("mu", C.c_double),
+                ("sigma", C.c_double)]
 
 
 class ServeStats(C.Structure):
@@ -490,7 +492,8 @@ def sweep(src: Job, ks, restarts: int, base_seed: int, nIters=2000, niter_test_c
         out.W_H_out = fac.ctypes.data_as(C.POINTER(C.c_float))
     _check(lib.nmfk_sweep_run(src.h, C.byref(cfg), C.byref(out)), "sweep")
     r = {"results": [res[i].as_dict() for i in range(nj)], "cls": cls, "consensus": cons,
-         "seconds": out.seconds, "tune_seconds": out.tune_seconds, "tuning": out.tuning.as_dict()}
+         "seconds": out.seconds, "tune_seconds": out.tune_seconds, "tuning": out.tuning.as_dict(),
+         "plan_target": out.plan_target}
     if want_factors:
         facs, o = [], 0
         for k in ks:
@@ -503,7 +506,7 @@ def sweep(src: Job, ks, restarts: int, base_seed: int, nIters=2000, niter_test_c
 
 
 def serve(requests, hbm_budget=200 << 30, capacity=1024, nIters=2000, niter_test_conv=10,
-          stop_threshold=40, mu=5.0, sigma=1.5, chunks_per_poll=1):
+          stop_threshold=40, mu=5.0, sigma=1.5, chunks_per_poll=1, plan_target=None):
     """nmfk_serve_run over [(n, m, k, restarts, seed, arrival_s), ...] (arrivals sorted).
     Returns (per-request result dicts, stats dict)."""
     lib = load()
@@ -512,6 +515,8 @@ def serve(requests, hbm_budget=200 << 30, capacity=1024, nIters=2000, niter_test
     cfg.hbm_budget, cfg.capacity = hbm_budget, capacity
     cfg.nIters, cfg.niter_test_conv, cfg.stop_threshold = nIters, niter_test_conv, stop_threshold
     cfg.mu, cfg.sigma, cfg.chunks_per_poll = mu, sigma, chunks_per_poll
+    if plan_target is not None:
+        cfg.plan_target = plan_target
     nr = len(requests)
     req = (Request * max(nr, 1))()
     for i, (n, m, k, r, seed, t) in enumerate(requests):

@@ -33,6 +33,7 @@ void nmfk_serve_default_config(nmfk_serve_config *c)
 	c->mu = 5.0;
 	c->sigma = 1.5;
 	c->chunks_per_poll = 1;
+	c->plan_target = 32;
 }
 
 /* Device bytes of one request: its own job (V, Vt and buffers) plus restarts - 1 jobs that
@@ -115,7 +116,10 @@ static int admit(const nmfk_serve_config *cfg, const nmfk_request *q, req_state 
 			e = NMFK_EINVAL;
 	}
 	for (uint32_t i = 0; i < q->restarts && !e; i++) {
-		e = nmfk_job_init_random(r->jobs[i], nmfk_sweep_seed(q->seed, q->k, i));
+		if (cfg->plan_target)
+			e = nmfk_job_replan(r->jobs[i], cfg->plan_target);
+		if (!e)
+			e = nmfk_job_init_random(r->jobs[i], nmfk_sweep_seed(q->seed, q->k, i));
 		if (!e && nmfk_batch_add(b, r->jobs[i], &opt) < 0)
 			e = NMFK_EINVAL;
 	}
@@ -242,9 +246,10 @@ int nmfk_serve_run(const nmfk_serve_config *cfg, const nmfk_request *req, uint32
 		}
 	}
 	const double t1 = nmfk_now() - t0;
+	/* The batch first: destroying it waits for its stream, which may still read jobs. */
+	nmfk_batch_destroy(b);
 	for (uint32_t i = 0; i < nreq; i++)
 		free_request(&rs[i], req[i].restarts);
-	nmfk_batch_destroy(b);
 	if (!e && stats) {
 		qsort(lat, finished, sizeof(double), cmp_double);
 		uint64_t jobs = 0;

@@ -90,6 +90,8 @@ int nmfk_sweep_run(nmfk_job *src, const nmfk_sweep_config *cfg, nmfk_sweep_out *
 	opt.stop_threshold = cfg->stop_threshold;
 	double tune_s = 0.0;
 	const double t0 = nmfk_now();
+	if (out)
+		out->plan_target = cfg->plan_target;
 	if (cfg->mode == NMFK_SWEEP_ONE_AT_A_TIME) {
 		for (uint32_t i = 0; i < nj && !e; i++) {
 			opt.k = jobs[i]->k;
@@ -99,8 +101,44 @@ int nmfk_sweep_run(nmfk_job *src, const nmfk_sweep_config *cfg, nmfk_sweep_out *
 				out->res[i] = r;
 		}
 	} else {
-		b = nmfk_batch_create(cfg->niter_test_conv, nj);
-		if (!b)
+		nmfk_batch_tuning best_tn;
+		nmfk_batch_default_tuning(&best_tn);
+		uint32_t best_target = cfg->plan_target;
+		if (cfg->autotune && cfg->plan_target == 0) {
+			/* Plan search: every candidate per-job plan (workgroups each job's partial
+			 * launch aims for; 0 = the process tunables) is tried on a trial batch of all
+			 * jobs, whose launch shape the batch autotuner picks; the fastest pair wins. */
+			const double ta = nmfk_now();
+			static const uint32_t cands[] = {8, 16, 32, 64, 0};
+			double best_ms = 1e30;
+			for (size_t c = 0; c < sizeof cands / sizeof cands[0] && !e; c++) {
+				for (uint32_t i = 0; i < nj && !e; i++)
+					e = nmfk_job_replan(jobs[i], cands[c]);
+				nmfk_batch *tb = e ? NULL : nmfk_batch_create(cfg->niter_test_conv, nj);
+				if (!e && !tb)
+					e = NMFK_EINVAL;
+				for (uint32_t i = 0; i < nj && !e; i++) {
+					opt.k = jobs[i]->k;
+					if (nmfk_batch_add(tb, jobs[i], &opt) < 0)
+						e = NMFK_EINVAL;
+				}
+				nmfk_batch_tuning tn;
+				double ms = 0.0;
+				if (!e)
+					e = nmfk_batch_autotune(tb, cfg->autotune_iters, &tn, &ms);
+				if (!e && ms < best_ms) {
+					best_ms = ms;
+					best_tn = tn;
+					best_target = cands[c];
+				}
+				nmfk_batch_destroy(tb);
+			}
+			for (uint32_t i = 0; i < nj && !e; i++)
+				e = nmfk_job_replan(jobs[i], best_target);
+			tune_s = nmfk_now() - ta;
+		}
+		b = e ? NULL : nmfk_batch_create(cfg->niter_test_conv, nj);
+		if (!e && !b)
 			e = NMFK_EINVAL;
 		for (uint32_t i = 0; i < nj && !e; i++) {
 			opt.k = jobs[i]->k;
@@ -108,13 +146,16 @@ int nmfk_sweep_run(nmfk_job *src, const nmfk_sweep_config *cfg, nmfk_sweep_out *
 				e = NMFK_EINVAL;
 		}
 		if (!e && cfg->autotune) {
-			const double ta = nmfk_now();
-			nmfk_batch_tuning best;
-			e = nmfk_batch_autotune(b, cfg->autotune_iters, &best, NULL);
-			tune_s = nmfk_now() - ta;
-			if (!e && out)
-				out->tuning = best;
+			if (cfg->plan_target == 0) {
+				e = nmfk_batch_set_tuning(b, &best_tn);
+			} else {
+				const double ta = nmfk_now();
+				e = nmfk_batch_autotune(b, cfg->autotune_iters, &best_tn, NULL);
+				tune_s = nmfk_now() - ta;
+			}
 		}
+		if (out)
+			out->plan_target = best_target;
 		nmfk_batch_tuning tn;
 		nmfk_batch_get_tuning(b, &tn);
 		nmfk_job **done = (nmfk_job **)malloc(nj * sizeof(nmfk_job *));

@@ -177,8 +177,10 @@ typedef struct nmfk_sweep_config {
 	uint64_t base_seed;
 	uint32_t nIters, niter_test_conv, stop_threshold;
 	uint32_t mode;            /* NMFK_SWEEP_BATCHED or NMFK_SWEEP_ONE_AT_A_TIME */
-	uint32_t plan_target;     /* partial-kernel workgroups each job's plan aims for (0: process tunables) */
-	uint32_t autotune;        /* batched mode: run nmfk_batch_autotune first */
+	uint32_t plan_target;     /* partial-kernel workgroups each job's plan aims for (0: process tunables,
+	                             or, with autotune, chosen by the plan search) */
+	uint32_t autotune;        /* batched mode: tune the launch shape first (and, when plan_target is 0,
+	                             the per-job plan among 8/16/32/64 workgroups and the tunables) */
 	uint32_t autotune_iters;
 } nmfk_sweep_config;
 
@@ -190,6 +192,7 @@ typedef struct nmfk_sweep_out {
 	double seconds;           /* wall time of the factorizations (job set-up and autotuning excluded) */
 	double tune_seconds;
 	nmfk_batch_tuning tuning; /* launch tuning used (batched mode) */
+	uint32_t plan_target;     /* per-job plan target used */
 } nmfk_sweep_out;
 
 void nmfk_sweep_default_config(nmfk_sweep_config *c);
@@ -226,6 +229,9 @@ typedef struct nmfk_serve_config {
 	uint32_t capacity;            /* jobs (restarts) advanced at once */
 	uint32_t nIters, niter_test_conv, stop_threshold;
 	uint32_t chunks_per_poll;
+	uint32_t plan_target;         /* partial-kernel workgroups each job's plan aims for (0: the
+	                                 single-job tunables); many jobs share the device, so fewer
+	                                 splits per job than alone (default 32) */
 	double mu, sigma;             /* V = exp(N(mu, sigma^2)) */
 } nmfk_serve_config;
 

@@ -567,7 +567,10 @@ __device__ __forceinline__ void partial_mfma_body(int bx, int by, const half_arg
 
 	// Full blocks stream through a D-deep register ring (D blocks in flight, slot indices
 	// compile-time after unrolling); a partial last block of the matrix is handled masked.
-	constexpr int D = KT <= 32 ? 3 : 1;   // KT 48/64: deeper rings cost occupancy
+	// Ring depth 2 up to KT 32: at KT 16 it fits 120 VGPRs (4 waves per SIMD, 9% faster than
+	// a 3-deep ring at 3 waves), at KT 32 166 VGPRs with the kernel's 3-wave bound (7% faster
+	// than 3 deep at 2 waves); KT 64 keeps one block in flight, deeper rings cost occupancy.
+	constexpr int D = KT <= 32 ? 2 : 1;
 	const int nfull = (b_end - b_begin) / 16;
 	// this wave group's full blocks: kb = wb, wb + NB, ...
 	const int mine = idle ? 0 : (nfull > wb ? (nfull - wb + NB - 1) / NB : 0);
@@ -662,7 +665,8 @@ __device__ __forceinline__ void partial_mfma_body(int bx, int by, const half_arg
 }
 
 template <int KT, int NA, int NB>
-__global__ __launch_bounds__(64 * NA * NB) void k_partial_mfma(half_args h, const nmfk_state *__restrict__ st, int S)
+__global__ __launch_bounds__(64 * NA * NB, (KT == 32 ? 3 : 1)) void k_partial_mfma(half_args h, const nmfk_state *__restrict__ st,
+										       int S)
 {
 	__shared__ float smem[smem_mfma(KT, NA, NB)];
 	if (st != nullptr && st->converged)

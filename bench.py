@@ -9,9 +9,13 @@ factorization (initialisation + 500 iterations) with V already resident in HBM.
 
 Prints one JSON line with the throughput, the HBM roofline of the dominant kernel (HIP
 events on the job's stream) and a CPU baseline (the oracle's scalar C rendering of the
-reference updates on the host cores, on a bounded sample of iterations).
+reference updates on the host cores, on a bounded sample of iterations).  The same line
+carries three more workloads: "sweep" (rank sweep V 5000 x 200, k = 2..10 x 20 restarts,
+batched against one job at a time), "dense_rank" (V 50000 x 2000, k = 64, BF16 against FP32
+contractions) and "served" (a Poisson stream of factorization requests, p50/p99 latency).
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline]
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline] [--no-extra]
+    python bench.py --served-only --served-requests 2000 --served-rate 4   # the full stream
 """
 from __future__ import annotations
 
@@ -127,7 +131,7 @@ def bench_served(pkg, count: int, rate: float) -> dict:
     (at most m), restarts U{1..10}, V lognormal(5, 1.5^2); HBM budget 200 GB; stop rule as
     the plain single factorization."""
     reqs = pkg.request_stream(count, rate, 2024)
-    out, st = pkg.serve(reqs)
+    out, st = pkg.serve(reqs, progress_every=max(1, count // 20))
     return {"workload": f"{count} requests, Poisson arrivals at {rate}/s, n log-U[1000,60000], "
                         "m log-U[20,3000], k U{2..min(32,m)}, restarts U{1..10}, HBM budget 200 GB",
             "requests_per_s": st["requests_per_s"], "factorizations_per_s": st["factorizations_per_s"],
@@ -148,10 +152,12 @@ def main():
                     help="skip the rank-sweep, dense-rank and served-stream measurements")
     ap.add_argument("--served-requests", type=int, default=100)
     ap.add_argument("--served-rate", type=float, default=5.0)
+    ap.add_argument("--served-only", action="store_true",
+                    help="run only the served stream (e.g. the full 2000-request stream) and print its line")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; the multi-GPU row-block "
-                                   "split is a design (see DESIGN.md), --gpus must be 1"}))
+        print(json.dumps({"error": "this build runs on one GPU only (the multi-GPU row-block split "
+                                   "of V is designed, not built); --gpus must be 1"}))
         sys.exit(2)
 
     pkg = load_package()
@@ -159,6 +165,12 @@ def main():
     if pkg.gpu_count() < 1:
         print(json.dumps({"error": "no HIP device visible"}))
         sys.exit(1)
+
+    if args.served_only:
+        s = bench_served(pkg, args.served_requests, args.served_rate)
+        print(json.dumps({"metric": "served stream: requests/s, p50/p99 latency", "value": s["requests_per_s"],
+                          "unit": "requests/s", "n_gpus": 1, "higher_is_better": True, "served": s}))
+        return
 
     job = pkg.Job(N, M, K)
     job.gen_V_planted(SEED_V, KSTAR, SIGMA)

@@ -104,8 +104,8 @@ class RequestResult(C.Structure):
 class ServeConfig(C.Structure):
     _fields_ = [("hbm_budget", C.c_size_t), ("capacity", C.c_uint32), ("nIters", C.c_uint32),
                 ("niter_test_conv", C.c_uint32), ("stop_threshold", C.c_uint32),
-                ("chunks_per_poll", C.c_uint32), ("plan_target", C.c_uint32), ("mu", C.c_double),
-                ("sigma", C.c_double)]
+                ("chunks_per_poll", C.c_uint32), ("plan_target", C.c_uint32), ("progress_every", C.c_uint32),
+                ("mu", C.c_double), ("sigma", C.c_double)]
 
 
 class ServeStats(C.Structure):
@@ -506,7 +506,7 @@ def sweep(src: Job, ks, restarts: int, base_seed: int, nIters=2000, niter_test_c
 
 
 def serve(requests, hbm_budget=200 << 30, capacity=1024, nIters=2000, niter_test_conv=10,
-          stop_threshold=40, mu=5.0, sigma=1.5, chunks_per_poll=1, plan_target=None):
+          stop_threshold=40, mu=5.0, sigma=1.5, chunks_per_poll=1, plan_target=None, progress_every=0):
     """nmfk_serve_run over [(n, m, k, restarts, seed, arrival_s), ...] (arrivals sorted).
     Returns (per-request result dicts, stats dict)."""
     lib = load()
@@ -517,6 +517,7 @@ def serve(requests, hbm_budget=200 << 30, capacity=1024, nIters=2000, niter_test
     cfg.mu, cfg.sigma, cfg.chunks_per_poll = mu, sigma, chunks_per_poll
     if plan_target is not None:
         cfg.plan_target = plan_target
+    cfg.progress_every = progress_every
     nr = len(requests)
     req = (Request * max(nr, 1))()
     for i, (n, m, k, r, seed, t) in enumerate(requests):

@@ -237,6 +237,11 @@ int nmfk_serve_run(const nmfk_serve_config *cfg, const nmfk_request *req, uint32
 				o->finished = nmfk_now() - t0;
 				o->latency = o->finished - req[qi].arrival;
 				lat[finished++] = o->latency;
+				if (cfg->progress_every && finished % cfg->progress_every == 0) {
+					fprintf(stderr, "nmfk_serve: %u/%u requests done at %.1f s (%u jobs active)\n", finished,
+						nreq, o->finished, active_jobs - req[qi].restarts);
+					fflush(stderr);
+				}
 				used -= rs[qi].bytes;
 				active_jobs -= req[qi].restarts;
 				free_request(&rs[qi], req[qi].restarts);

@@ -154,6 +154,8 @@ def main():
     ap.add_argument("--served-rate", type=float, default=5.0)
     ap.add_argument("--served-only", action="store_true",
                     help="run only the served stream (e.g. the full 2000-request stream) and print its line")
+    ap.add_argument("--dense-only", action="store_true",
+                    help="run only the dense-rank (k = 64, BF16 vs FP32) workload and print its line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only (the multi-GPU row-block split "
@@ -166,6 +168,12 @@ def main():
         print(json.dumps({"error": "no HIP device visible"}))
         sys.exit(1)
 
+    if args.dense_only:
+        d = bench_dense_rank(pkg)
+        print(json.dumps({"metric": "dense rank: factorizations/s (BF16 contractions)",
+                          "value": d["bf16_factorizations_per_s"], "unit": "factorizations/s", "n_gpus": 1,
+                          "higher_is_better": True, "dense_rank": d}))
+        return
     if args.served_only:
         s = bench_served(pkg, args.served_requests, args.served_rate)
         print(json.dumps({"metric": "served stream: requests/s, p50/p99 latency", "value": s["requests_per_s"],

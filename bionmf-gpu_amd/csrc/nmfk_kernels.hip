@@ -263,39 +263,52 @@ __device__ __forceinline__ void partial_valu_body(int bx, int by, const half_arg
 	const int b_begin = by * bchunk;
 	const int b_end = min(q, b_begin + bchunk);
 	const float *Ucol = U + (avail > 0 ? a0 : 0);
-	// One row of U is prefetched ahead so its latency overlaps the current row's math.
-	int b = b_begin + wave;
-	float un[AP];
-#pragma unroll
-	for (int r = 0; r < AP; r++)
-		un[r] = 0.0f;
-	if (b < b_end)
-		load_u<AP>(Ucol + (size_t)b * ldu, avail, un);
-	for (; b < b_end; b += NW) {
-		float u[AP];
-#pragma unroll
-		for (int r = 0; r < AP; r++)
-			u[r] = un[r];
-		if (b + NW < b_end)
-			load_u<AP>(Ucol + (size_t)(b + NW) * ldu, avail, un);
-		const float *Bb = B + (size_t)b * KT;
-		float bv[KT];
-#pragma unroll
-		for (int c = 0; c < KT; c++)
-			bv[c] = Bb[c];
-#pragma unroll
-		for (int r = 0; r < AP; r++) {
-			float wh = 0.0f;
+	// The next DR rows of this wave (U values and the B row) are in flight while a row is
+	// computed; a consumed slot is refilled with the row DR steps ahead.  Rows are still
+	// accumulated in ascending order.  (DR = 2 measured as fast as 4, with fewer registers,
+	// and 13% faster than one row ahead at 20000 x 1000, k = 3.)
+	constexpr int DR = 2;
+	float un[DR][AP], bn[DR][KT];
+	auto fetch = [&](int rb, float (&u)[AP], float (&bv)[KT]) {
+		if (rb < b_end) {
+			load_u<AP>(Ucol + (size_t)rb * ldu, avail, u);
+			const float *Bb = B + (size_t)rb * KT;
 #pragma unroll
 			for (int c = 0; c < KT; c++)
-				wh = fmaf(av[r][c], bv[c], wh);
-			// ratio = U / WH with the hardware reciprocal (<= 1 ulp) and one rounding of the
-			// product, the fast division the reference's default build uses
-			// (include/real_type.h:283-288, --use_fast_math).
-			const float ratio = u[r] * __builtin_amdgcn_rcpf(wh);
+				bv[c] = Bb[c];
+		}
+	};
+#pragma unroll
+	for (int d = 0; d < DR; d++)
+		fetch(b_begin + wave + d * NW, un[d], bn[d]);
+	for (int b = b_begin + wave; b < b_end; b += DR * NW) {
+#pragma unroll
+		for (int d = 0; d < DR; d++) {
+			const int rb = b + d * NW;
+			if (rb >= b_end)
+				break;
+			float u[AP], bv[KT];
+#pragma unroll
+			for (int r = 0; r < AP; r++)
+				u[r] = un[d][r];
 #pragma unroll
 			for (int c = 0; c < KT; c++)
-				acc[r][c] = fmaf(ratio, bv[c], acc[r][c]);
+				bv[c] = bn[d][c];
+			fetch(rb + DR * NW, un[d], bn[d]);
+#pragma unroll
+			for (int r = 0; r < AP; r++) {
+				float wh = 0.0f;
+#pragma unroll
+				for (int c = 0; c < KT; c++)
+					wh = fmaf(av[r][c], bv[c], wh);
+				// ratio = U / WH with the hardware reciprocal (<= 1 ulp) and one rounding of
+				// the product, the fast division the reference's default build uses
+				// (include/real_type.h:283-288, --use_fast_math).
+				const float ratio = u[r] * __builtin_amdgcn_rcpf(wh);
+#pragma unroll
+				for (int c = 0; c < KT; c++)
+					acc[r][c] = fmaf(ratio, bv[c], acc[r][c]);
+			}
 		}
 	}
 

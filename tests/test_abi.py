@@ -133,3 +133,31 @@ def test_get_padding_matches_reference(libs):
         lib.set_default_matrix_limits()
     for d in (1, 2, 15, 16, 17, 38, 1000, 4097):
         assert prod.get_padding(d) == ref.get_padding(d)
+
+
+def test_exports_cover_the_reference_driver(pkg):
+    """Every function and global the reference's src/NMF_GPU.c uses from the headers this
+    library replaces (NMF_routines.h, matrix_operations.h, GPU_setup.h, common.h) is
+    exported, so that driver links against libnmfgpu_amd.so (its matrix I/O comes from the
+    reference's own ISO-C sources)."""
+    import re
+    import subprocess
+    ref = pathlib.Path("/root/reference")
+    if not (ref / "src" / "NMF_GPU.c").exists():
+        pytest.skip("reference tree absent")
+
+    def strip(t):
+        return re.sub(r"//[^\n]*", "", re.sub(r"/\*.*?\*/", "", t, flags=re.S))
+    src = strip((ref / "src" / "NMF_GPU.c").read_text(errors="replace"))
+    hdrs = strip("".join((ref / "include" / h).read_text(errors="replace")
+                         for h in ("NMF_routines.h", "matrix_operations.h", "GPU_setup.h", "common.h")))
+    keywords = {"if", "for", "while", "switch", "return", "sizeof"}
+    declared = set(re.findall(r"\b([A-Za-z_]\w*)\s*\(", hdrs)) - keywords
+    used = {c for c in re.findall(r"\b([A-Za-z_]\w*)\s*\(", src) if c in declared and not c.isupper()}
+    glob = set(re.findall(r"\bextern\s+[^;(]*?\b([A-Za-z_]\w*)\s*(?:\[[^\]]*\])?\s*;", hdrs))
+    used |= {g for g in glob if re.search(r"\b" + g + r"\b", src)}
+    out = subprocess.run(["nm", "-D", "--defined-only", str(pkg.LIB_PATH)], capture_output=True, text=True,
+                         check=True).stdout
+    syms = {line.split()[-1] for line in out.splitlines() if line.strip()}
+    assert len(used) > 20
+    assert sorted(used - syms) == []

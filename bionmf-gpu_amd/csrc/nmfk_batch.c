@@ -21,7 +21,13 @@
 #include <stdlib.h>
 #include <string.h>
 
-enum { TAB_PART_H, TAB_PART_W, TAB_FIN_H, TAB_FIN_W, TAB_TEST, NTAB };
+/* Partial tables per half and body class (VALU jobs kt <= 8, matrix-core jobs kt 16/32). */
+enum { TAB_PART_H0, TAB_PART_H1, TAB_PART_W0, TAB_PART_W1, TAB_FIN_H, TAB_FIN_W, TAB_TEST, NTAB };
+
+static int job_cls(const nmfk_job *j)
+{
+	return j->kt <= 8 ? 0 : 1;
+}
 
 struct nmfk_batch {
 	uint32_t T, cap, nslots;      /* nslots: highest slot in use + 1 */
@@ -224,13 +230,14 @@ static void push(nmfk_tile *t, uint32_t *n, uint32_t job, uint32_t bx, uint32_t 
 	(*n)++;
 }
 
-/* Partial tiles of one half for every live job.  order 0: job by job; order 1: split by
- * split across jobs (round-robin), so concurrently running workgroups belong to many jobs. */
-static void push_partial(nmfk_batch *b, nmfk_tile *t, uint32_t *n, int half)
+/* Partial tiles of one half for every live job of body class cls.  order 0: job by job;
+ * order 1: split by split across jobs (round-robin), so concurrently running workgroups
+ * belong to many jobs. */
+static void push_partial(nmfk_batch *b, nmfk_tile *t, uint32_t *n, int half, int cls)
 {
 	uint32_t maxS = 0;
 	for (uint32_t s = 0; s < b->nslots; s++)
-		if (b->job[s]) {
+		if (b->job[s] && job_cls(b->job[s]) == cls) {
 			const nmfk_hplan *p = half ? &b->job[s]->planW : &b->job[s]->planH;
 			push(t, n, s, p->nab, 0);   /* denominator tile first: the finish needs it */
 			if (p->S > maxS)
@@ -238,7 +245,7 @@ static void push_partial(nmfk_batch *b, nmfk_tile *t, uint32_t *n, int half)
 		}
 	if (b->tune.order == 0) {
 		for (uint32_t s = 0; s < b->nslots; s++)
-			if (b->job[s]) {
+			if (b->job[s] && job_cls(b->job[s]) == cls) {
 				const nmfk_hplan *p = half ? &b->job[s]->planW : &b->job[s]->planH;
 				for (uint32_t by = 0; by < p->S; by++)
 					for (uint32_t bx = 0; bx < p->nab; bx++)
@@ -247,7 +254,7 @@ static void push_partial(nmfk_batch *b, nmfk_tile *t, uint32_t *n, int half)
 	} else {
 		for (uint32_t by = 0; by < maxS; by++)
 			for (uint32_t s = 0; s < b->nslots; s++)
-				if (b->job[s]) {
+				if (b->job[s] && job_cls(b->job[s]) == cls) {
 					const nmfk_hplan *p = half ? &b->job[s]->planW : &b->job[s]->planH;
 					if (by < p->S)
 						for (uint32_t bx = 0; bx < p->nab; bx++)
@@ -278,12 +285,13 @@ static int build_tables(nmfk_batch *b)
 		b->tiles_cap = need;
 	}
 	uint32_t n = 0;
-	b->off[TAB_PART_H] = n;
-	push_partial(b, b->h_tiles, &n, 0);
-	b->cnt[TAB_PART_H] = n - b->off[TAB_PART_H];
-	b->off[TAB_PART_W] = n;
-	push_partial(b, b->h_tiles, &n, 1);
-	b->cnt[TAB_PART_W] = n - b->off[TAB_PART_W];
+	for (int half = 0; half < 2; half++)
+		for (int cls = 0; cls < 2; cls++) {
+			const int tab = TAB_PART_H0 + 2 * half + cls;
+			b->off[tab] = n;
+			push_partial(b, b->h_tiles, &n, half, cls);
+			b->cnt[tab] = n - b->off[tab];
+		}
 	b->off[TAB_FIN_H] = n;
 	for (uint32_t s = 0; s < b->nslots; s++)
 		if (b->job[s])
@@ -338,14 +346,16 @@ static int build_tables(nmfk_batch *b)
 static int enqueue_iteration(nmfk_batch *b, int clamp)
 {
 	const nmfk_tile *t = b->d_tiles;
-	int e = nmfk_launch_batch_partial(b->d_desc, t + b->off[TAB_PART_H], b->cnt[TAB_PART_H], 0,
-					  b->tune.grid_partial, b->stream);
+	int e = 0;
+	for (int cls = 0; cls < 2 && !e; cls++)
+		e = nmfk_launch_batch_partial(b->d_desc, t + b->off[TAB_PART_H0 + cls], b->cnt[TAB_PART_H0 + cls], 0,
+					      cls, b->tune.grid_partial, b->stream);
 	if (!e)
 		e = nmfk_launch_batch_finish(b->d_desc, t + b->off[TAB_FIN_H], b->cnt[TAB_FIN_H], 0, 0,
 					     b->tune.grid_finish, b->stream);
-	if (!e)
-		e = nmfk_launch_batch_partial(b->d_desc, t + b->off[TAB_PART_W], b->cnt[TAB_PART_W], 1,
-					      b->tune.grid_partial, b->stream);
+	for (int cls = 0; cls < 2 && !e; cls++)
+		e = nmfk_launch_batch_partial(b->d_desc, t + b->off[TAB_PART_W0 + cls], b->cnt[TAB_PART_W0 + cls], 1,
+					      cls, b->tune.grid_partial, b->stream);
 	if (!e)
 		e = nmfk_launch_batch_finish(b->d_desc, t + b->off[TAB_FIN_W], b->cnt[TAB_FIN_W], 1, clamp,
 					     b->tune.grid_finish, b->stream);

@@ -154,9 +154,10 @@ typedef struct nmfk_tile {
 /* Largest padded rank a batched launch handles (its LDS is sized for it). */
 #define NMFK_BATCH_KTMAX 32
 
-/* half: 0 = H half (U = V, A = H, B = W), 1 = W half (U = Vt, A = W, B = H). */
+/* half: 0 = H half (U = V, A = H, B = W), 1 = W half (U = Vt, A = W, B = H).  cls: the body
+ * class of every tile's job, 0 = vector ALUs (kt 4/8), 1 = matrix cores (kt 16/32). */
 int nmfk_launch_batch_partial(const nmfk_bjob *jobs, const nmfk_tile *tiles, uint32_t ntiles, int half,
-			      uint32_t grid, hipStream_t stream);
+			      int cls, uint32_t grid, hipStream_t stream);
 int nmfk_launch_batch_finish(const nmfk_bjob *jobs, const nmfk_tile *tiles, uint32_t ntiles, int half,
 			     int clamp, uint32_t grid, hipStream_t stream);
 /* Test: rows tiles (job, row block of NMFK_TEST_ROWS), then one final workgroup per job. */

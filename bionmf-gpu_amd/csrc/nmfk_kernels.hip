@@ -1383,35 +1383,36 @@ __device__ __forceinline__ half_args batch_half(const nmfk_bjob &J, int half)
 	return h;
 }
 
+// Two batched partial kernels, one per body class, so each runs at its own occupancy:
+// VALU bodies (KT 4/8, <= 136 VGPRs) and matrix-core bodies (KT 16/32); a batch launches
+// both (each over its own tile table).
+template <int CLS>
 constexpr int smem_batch_partial()
 {
-	return smem_valu(8, 4, 4) > smem_mfma(NMFK_BATCH_KTMAX, 1, 4) ? smem_valu(8, 4, 4)
-								       : smem_mfma(NMFK_BATCH_KTMAX, 1, 4);
+	return CLS == 0 ? smem_valu(8, 4, 4) : smem_mfma(NMFK_BATCH_KTMAX, 1, 4);
 }
 
+template <int CLS>
 __global__ __launch_bounds__(NMFK_WG) void k_batch_partial(const nmfk_bjob *__restrict__ jobs,
 							   const nmfk_tile *__restrict__ tiles, int ntiles, int half)
 {
-	__shared__ float smem[smem_batch_partial()];
+	__shared__ float smem[smem_batch_partial<CLS>()];
 	for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
 		const nmfk_tile tl = tiles[t];
 		const nmfk_bjob &J = jobs[tl.job];
 		if (!J.live || J.st->converged)
 			continue;
 		const half_args h = batch_half(J, half);
-		switch (J.kt) {
-		case 4:
-			partial_valu_body<4, 4, 4>(tl.bx, tl.by, h, smem);
-			break;
-		case 8:
-			partial_valu_body<8, 4, 4>(tl.bx, tl.by, h, smem);
-			break;
-		case 16:
-			partial_mfma_body<16, 1, 4>(tl.bx, tl.by, h, smem);
-			break;
-		case 32:
-			partial_mfma_body<32, 1, 4>(tl.bx, tl.by, h, smem);
-			break;
+		if constexpr (CLS == 0) {
+			if (J.kt == 4)
+				partial_valu_body<4, 4, 4>(tl.bx, tl.by, h, smem);
+			else
+				partial_valu_body<8, 4, 4>(tl.bx, tl.by, h, smem);
+		} else {
+			if (J.kt == 16)
+				partial_mfma_body<16, 1, 4>(tl.bx, tl.by, h, smem);
+			else
+				partial_mfma_body<32, 1, 4>(tl.bx, tl.by, h, smem);
 		}
 		__syncthreads();   // the next tile reuses the LDS
 	}
@@ -2023,12 +2024,16 @@ static inline unsigned batch_grid(uint32_t ntiles, uint32_t grid)
 }
 
 extern "C" int nmfk_launch_batch_partial(const nmfk_bjob *jobs, const nmfk_tile *tiles, uint32_t ntiles, int half,
-					 uint32_t grid, hipStream_t stream)
+					 int cls, uint32_t grid, hipStream_t stream)
 {
 	if (ntiles == 0)
 		return 0;
-	hipLaunchKernelGGL(k_batch_partial, dim3(batch_grid(ntiles, grid)), dim3(NMFK_WG), 0, stream, jobs, tiles,
-			   (int)ntiles, half);
+	if (cls == 0)
+		hipLaunchKernelGGL((k_batch_partial<0>), dim3(batch_grid(ntiles, grid)), dim3(NMFK_WG), 0, stream, jobs,
+				   tiles, (int)ntiles, half);
+	else
+		hipLaunchKernelGGL((k_batch_partial<1>), dim3(batch_grid(ntiles, grid)), dim3(NMFK_WG), 0, stream, jobs,
+				   tiles, (int)ntiles, half);
 	return (int)hipGetLastError();
 }
 

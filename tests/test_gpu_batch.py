@@ -182,3 +182,24 @@ def test_sweep_seed_is_splitmix(nmfk):
         return z ^ (z >> 31)
     for base, k, r in [(0, 0, 0), (1234, 5, 7), (2**63 + 5, 10, 19)]:
         assert nmfk.sweep_seed(base, k, r) == splitmix(base, k, r)
+
+
+def test_full_sweep_matches_fp64(nmfk, orc):
+    # The benchmark's rank sweep at full shape (V 5000 x 200 planted rank 6, k = 2..10 x 20
+    # restarts, batched with the autotuner); restart 0 of every rank against the FP64 oracle:
+    # stop iteration and class vector exact, KL within 1e-3.
+    n, m, ks, R, base = 5000, 200, list(range(2, 11)), 20, 4242
+    V = orc.gen_planted(n, m, 6, 777, 0.3)
+    src = nmfk.Job(n, m, 1)
+    src.set_V(V)
+    a = nmfk.sweep(src, ks, R, base, autotune=True, autotune_iters=5, want_factors=True)
+    for q, k in enumerate(ks):
+        i = q * R
+        W0, H0 = orc.init_WH(n, m, k, nmfk.sweep_seed(base, k, 0))
+        o = orc.nmf(V, W0, H0, 2000, 10, 40, fp64=True)
+        assert a["results"][i]["iterations"] == o["iterations"], k
+        assert np.array_equal(a["cls"][q][0], orc.idx_max(o["H"])), k
+        W, H = a["factors"][i]
+        kl, klo = orc.kl_div(V, W, H), orc.kl_div(V, o["W"], o["H"])
+        assert abs(kl - klo) / klo < 1e-3, k
+    src.close()

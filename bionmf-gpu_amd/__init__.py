@@ -179,6 +179,10 @@ def load(path: os.PathLike | str | None = None):
     _sig(lib, "nmfk_job_create_shared", vp, [vp, u32])
     _sig(lib, "nmfk_job_replan", i32, [vp, u32])
     _sig(lib, "nmfk_job_plan_get", i32, [vp, _U32P])
+    _sig(lib, "nmfk_job_create_streamed", vp, [u32, u32, u32, _F32P, sz, u32, i32])
+    _sig(lib, "nmfk_job_streamed_info", i32, [vp, C.POINTER(u32), C.POINTER(u32), C.POINTER(dbl)])
+    _sig(lib, "nmfk_host_alloc", vp, [sz])
+    _sig(lib, "nmfk_host_free", None, [vp])
     _sig(lib, "nmfk_batch_default_tuning", None, [C.POINTER(BatchTuning)])
     _sig(lib, "nmfk_batch_create", vp, [u32, u32])
     _sig(lib, "nmfk_batch_destroy", None, [vp])
@@ -252,10 +256,35 @@ def default_options(k: int = 2, nIters: int = 2000, niter_test_conv: int = 10,
     return Options(k, nIters, niter_test_conv, stop_threshold)
 
 
+class PinnedArray:
+    """Pinned host memory (nmfk_host_alloc) seen as a C-contiguous float32 array."""
+
+    def __init__(self, shape):
+        self.lib = load()
+        count = int(np.prod(shape))
+        self.ptr = self.lib.nmfk_host_alloc(4 * count)
+        if not self.ptr:
+            raise NmfkError(f"nmfk_host_alloc failed: {last_error()}")
+        self.array = np.ctypeslib.as_array((C.c_float * count).from_address(self.ptr)).reshape(shape)
+
+    def close(self):
+        if self.ptr:
+            self.array = None
+            self.lib.nmfk_host_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Job:
     """One factorization V (n x m) ~ W (n x k) . H^T (H is m x k), resident on the GPU.
 
-    Job(n, m, k) owns its V; Job.shared(src, k) reads the V of <src> without a copy."""
+    Job(n, m, k) owns its V; Job.shared(src, k) reads the V of <src> without a copy;
+    Job.streamed(V, k) keeps V in host memory and streams it in row blocks."""
 
     def __init__(self, n: int, m: int, k: int, _src: "Job | None" = None):
         self.lib = load()
@@ -272,6 +301,34 @@ class Job:
     def shared(cls, src: "Job", k: int) -> "Job":
         return cls(src.n, src.m, k, _src=src)
 
+    @classmethod
+    def streamed(cls, V: np.ndarray, k: int, block_rows: int = 0, pinned: bool = False) -> "Job":
+        """nmfk_job_create_streamed: V (n x m) stays in host memory, copied to the device in
+        row blocks of <block_rows> rows (a multiple of 64; 0 = library default) once per
+        iteration.  pinned=True first copies V into pinned memory that the job keeps."""
+        self = cls.__new__(cls)
+        self.lib = load()
+        V = np.ascontiguousarray(V, dtype=np.float32)
+        self.n, self.m = (int(x) for x in V.shape)
+        self.k, self._src, self._pin = int(k), None, None
+        if pinned:
+            self._pin = PinnedArray(V.shape)
+            self._pin.array[...] = V
+            V = self._pin.array
+        self._hostV = V
+        self.h = self.lib.nmfk_job_create_streamed(self.n, self.m, self.k, V, self.m, block_rows, int(pinned))
+        if not self.h:
+            self._hostV = None
+            if self._pin is not None:
+                self._pin.close()
+            raise NmfkError(f"nmfk_job_create_streamed failed: {last_error()}")
+        return self
+
+    def streamed_info(self) -> dict:
+        br, nb, by = C.c_uint32(), C.c_uint32(), C.c_double()
+        _check(self.lib.nmfk_job_streamed_info(self.h, C.byref(br), C.byref(nb), C.byref(by)), "streamed_info")
+        return {"block_rows": br.value, "nblocks": nb.value, "bytes_copied": by.value}
+
     def replan(self, target_wgs: int):
         _check(self.lib.nmfk_job_replan(self.h, target_wgs), "replan")
 
@@ -285,6 +342,11 @@ class Job:
         if self.h:
             self.lib.nmfk_job_destroy(self.h)
             self.h = None
+        pin = getattr(self, "_pin", None)
+        if pin is not None:         # after the job: its streams no longer read the buffer
+            self._hostV = None
+            pin.close()
+            self._pin = None
 
     def __del__(self):
         try:

@@ -176,6 +176,10 @@ int nmfk_batch_add(nmfk_batch *b, nmfk_job *j, const nmfk_options *opt)
 			       "with nIters >= niter_test_conv");
 		return -1;
 	}
+	if (j->streamed) {
+		nmfk_set_error("nmfk_batch_add: a streamed job's V stays in host memory; batched jobs need it on the device");
+		return -1;
+	}
 	if (j->kt > NMFK_BATCH_KTMAX || !plan_batchable(&j->planH, j->kt) || !plan_batchable(&j->planW, j->kt)) {
 		nmfk_set_error("nmfk_batch_add: job (k=%u) has a kernel plan the batched launches do not run "
 			       "(batched: k <= %d on the default kernel paths)", j->k, NMFK_BATCH_KTMAX);

@@ -125,6 +125,55 @@ static int dalloc(void **p, size_t bytes, const char *what)
 	return nmfk_hip_fail(hipMalloc(p, bytes), what);
 }
 
+/* Jobs whose device V can be written: not shared (nmfk_job_create_shared), not streamed. */
+static int v_writable(const nmfk_job *j)
+{
+	if (j->streamed) {
+		nmfk_set_error("a streamed job's V stays in host memory (nmfk_job_create_streamed)");
+		return 0;
+	}
+	if (!j->owns_V) {
+		nmfk_set_error("V of a job created by nmfk_job_create_shared belongs to its source job");
+		return 0;
+	}
+	return 1;
+}
+
+/* Operations that need V on the device. */
+static int needs_device_V(const nmfk_job *j, const char *what)
+{
+	if (j->streamed) {
+		nmfk_set_error("%s: not available for a streamed job (V stays in host memory)", what);
+		return NMFK_EINVAL;
+	}
+	return 0;
+}
+
+static int finish_V(nmfk_job *j)
+{
+	j->colW_valid = 0;
+	j->vt_stale = 0;
+	int e = nmfk_launch_transpose(j->V, j->n, j->m, j->ldv, j->Vt, j->ldvt, j->stream);
+	if (e)
+		return nmfk_hip_fail(e, "transpose V");
+	return nmfk_hip_fail(hipStreamSynchronize(j->stream), "set V");
+}
+
+int nmfk_job_mark_V_changed(nmfk_job *j)
+{
+	j->vt_stale = 1;
+	return 0;
+}
+
+int nmfk_job_refresh_Vt(nmfk_job *j)
+{
+	if (!j->vt_stale)
+		return 0;
+	j->vt_stale = 0;
+	return nmfk_hip_fail(nmfk_launch_transpose(j->V, j->n, j->m, j->ldv, j->Vt, j->ldvt, j->stream),
+			     "transpose V");
+}
+
 static nmfk_job *job_create(uint32_t n, uint32_t m, uint32_t k, nmfk_job *src);
 
 nmfk_job *nmfk_job_create(uint32_t n, uint32_t m, uint32_t k)
@@ -136,6 +185,10 @@ nmfk_job *nmfk_job_create_shared(nmfk_job *src, uint32_t k)
 {
 	if (!src) {
 		nmfk_set_error("nmfk_job_create_shared: no source job");
+		return NULL;
+	}
+	if (src->streamed) {
+		nmfk_set_error("nmfk_job_create_shared: the source job's V is in host memory (streamed job)");
 		return NULL;
 	}
 	if (nmfk_job_refresh_Vt(src) || nmfk_job_sync(src))
@@ -233,6 +286,8 @@ void nmfk_job_destroy(nmfk_job *j)
 		hipStreamSynchronize(j->stream);
 	if (j->chunk_exec)
 		hipGraphExecDestroy(j->chunk_exec);
+	if (j->streamed)
+		nmfk_stream_free(j);
 	if (j->owns_V)
 		hipFree(j->V), hipFree(j->Vt);
 	hipFree(j->W), hipFree(j->H), hipFree(j->part);
@@ -261,6 +316,8 @@ int nmfk_job_info_get(const nmfk_job *j, nmfk_job_info *info)
 
 int nmfk_job_set_kernel_path(nmfk_job *j, int path)
 {
+	if (needs_device_V(j, "nmfk_job_set_kernel_path"))
+		return NMFK_EINVAL;
 	if (path < NMFK_PATH_AUTO || path > NMFK_PATH_MFMA_BF16) {
 		nmfk_set_error("nmfk_job_set_kernel_path: unknown path %d", path);
 		return NMFK_EINVAL;
@@ -284,6 +341,8 @@ int nmfk_job_set_kernel_path(nmfk_job *j, int path)
 
 int nmfk_job_replan(nmfk_job *j, uint32_t target_wgs)
 {
+	if (needs_device_V(j, "nmfk_job_replan"))
+		return NMFK_EINVAL;
 	nmfk_hplan h, w;
 	nmfk_plan_half_target(j->m, j->n, j->kt, (int)j->planH.path, target_wgs, &h);
 	nmfk_plan_half_target(j->n, j->m, j->kt, (int)j->planW.path, target_wgs, &w);
@@ -319,37 +378,10 @@ int nmfk_job_sync(nmfk_job *j)
 
 /* ---------------------------------------------------------------- inputs */
 
-static int finish_V(nmfk_job *j)
-{
-	j->colW_valid = 0;
-	j->vt_stale = 0;
-	int e = nmfk_launch_transpose(j->V, j->n, j->m, j->ldv, j->Vt, j->ldvt, j->stream);
-	if (e)
-		return nmfk_hip_fail(e, "transpose V");
-	return nmfk_hip_fail(hipStreamSynchronize(j->stream), "set V");
-}
-
-int nmfk_job_mark_V_changed(nmfk_job *j)
-{
-	j->vt_stale = 1;
-	return 0;
-}
-
-int nmfk_job_refresh_Vt(nmfk_job *j)
-{
-	if (!j->vt_stale)
-		return 0;
-	j->vt_stale = 0;
-	return nmfk_hip_fail(nmfk_launch_transpose(j->V, j->n, j->m, j->ldv, j->Vt, j->ldvt, j->stream),
-			     "transpose V");
-}
-
 int nmfk_job_set_V(nmfk_job *j, const float *V, size_t ldv)
 {
-	if (!j->owns_V) {
-		nmfk_set_error("V of a job created by nmfk_job_create_shared belongs to its source job");
+	if (!v_writable(j))
 		return NMFK_EINVAL;
-	}
 	if (ldv < j->m) {
 		nmfk_set_error("nmfk_job_set_V: ldv=%zu < m=%u", ldv, j->m);
 		return NMFK_EINVAL;
@@ -363,10 +395,8 @@ int nmfk_job_set_V(nmfk_job *j, const float *V, size_t ldv)
 
 int nmfk_job_set_V_device(nmfk_job *j, const float *dV, size_t ldv)
 {
-	if (!j->owns_V) {
-		nmfk_set_error("V of a job created by nmfk_job_create_shared belongs to its source job");
+	if (!v_writable(j))
 		return NMFK_EINVAL;
-	}
 	if (ldv < j->m) {
 		nmfk_set_error("nmfk_job_set_V_device: ldv=%zu < m=%u", ldv, j->m);
 		return NMFK_EINVAL;
@@ -380,10 +410,8 @@ int nmfk_job_set_V_device(nmfk_job *j, const float *dV, size_t ldv)
 
 int nmfk_job_gen_V_lognormal(nmfk_job *j, uint64_t seed, double mu, double sigma)
 {
-	if (!j->owns_V) {
-		nmfk_set_error("V of a job created by nmfk_job_create_shared belongs to its source job");
+	if (!v_writable(j))
 		return NMFK_EINVAL;
-	}
 	int e = nmfk_launch_gen_lognormal(j->V, j->n, j->m, (uint32_t)j->ldv, seed, mu, sigma, j->stream);
 	if (e)
 		return nmfk_hip_fail(e, "gen lognormal");
@@ -392,10 +420,8 @@ int nmfk_job_gen_V_lognormal(nmfk_job *j, uint64_t seed, double mu, double sigma
 
 int nmfk_job_gen_V_planted(nmfk_job *j, uint64_t seed, uint32_t kstar, double sigma)
 {
-	if (!j->owns_V) {
-		nmfk_set_error("V of a job created by nmfk_job_create_shared belongs to its source job");
+	if (!v_writable(j))
 		return NMFK_EINVAL;
-	}
 	double *scratch = NULL;
 	int e = dalloc((void **)&scratch, 8 * (size_t)kstar * ((size_t)j->n + j->m), "hipMalloc(planted)");
 	if (e)
@@ -412,6 +438,8 @@ int nmfk_job_gen_V_planted(nmfk_job *j, uint64_t seed, uint32_t kstar, double si
 
 int nmfk_job_get_V(nmfk_job *j, float *V, size_t ldv)
 {
+	if (needs_device_V(j, "nmfk_job_get_V"))
+		return NMFK_EINVAL;
 	if (ldv < j->m) {
 		nmfk_set_error("nmfk_job_get_V: ldv < m");
 		return NMFK_EINVAL;
@@ -544,6 +572,8 @@ int nmfk_job_shadow_for_W(nmfk_job *j)
 /* update_H(): accum_W = colsum(W) (src/NMF_routines.c:720), then the H half-step. */
 int nmfk_job_update_H(nmfk_job *j)
 {
+	if (needs_device_V(j, "nmfk_job_update_H"))
+		return NMFK_EINVAL;
 	int e = nmfk_launch_colsum_parts(j->W, j->n, j->kt, j->colW, j->stream);
 	if (!e)
 		e = nmfk_job_shadow_for_H(j);
@@ -555,6 +585,8 @@ int nmfk_job_update_H(nmfk_job *j)
 /* update_W(): accum_H = colsum(H) (src/NMF_routines.c:1360), then the W half-step. */
 int nmfk_job_update_W(nmfk_job *j)
 {
+	if (needs_device_V(j, "nmfk_job_update_W"))
+		return NMFK_EINVAL;
 	if (nmfk_job_refresh_Vt(j))
 		return NMFK_EHIP;
 	int e = nmfk_launch_colsum_parts(j->H, j->m, j->kt, j->colH, j->stream);
@@ -632,6 +664,8 @@ int nmfk_job_run(nmfk_job *j, const nmfk_options *opt, nmfk_result *res)
 		nmfk_set_error("nmfk_job_run: options do not match the job (k=%u, niter_test_conv>0)", j->k);
 		return NMFK_EINVAL;
 	}
+	if (j->streamed)
+		return nmfk_stream_run(j, opt, res);
 	const uint32_t T = opt->niter_test_conv;
 	const uint32_t niter_conv = opt->nIters / T, niter_rem = opt->nIters % T;
 	if (nmfk_job_refresh_Vt(j))
@@ -701,6 +735,8 @@ int nmfk_job_run(nmfk_job *j, const nmfk_options *opt, nmfk_result *res)
 
 int nmfk_job_dot_kl(nmfk_job *j, double *dot_V, double *dot_VWH, double *kl)
 {
+	if (j->streamed)
+		return nmfk_stream_dot_kl(j, dot_V, dot_VWH, kl);
 	double *h = (double *)malloc(sizeof(double) * 3 * j->ndot);
 	if (!h) {
 		nmfk_set_error("out of host memory");
@@ -830,6 +866,8 @@ int nmfk_measure_stream(size_t bytes, uint32_t iters, uint32_t blocks, double *r
 
 int nmfk_job_time_iterations(nmfk_job *j, uint32_t iters, nmfk_timing *t)
 {
+	if (needs_device_V(j, "nmfk_job_time_iterations"))
+		return NMFK_EINVAL;
 	int e = nmfk_launch_colsum_parts(j->W, j->n, j->kt, j->colW, j->stream);
 	if (!e)
 		e = nmfk_job_shadow_for_H(j);

@@ -42,7 +42,30 @@ struct nmfk_job {
 	 * (rows x kt) and transposed (kt x ldt, ldt = rows rounded up to 32, zero padding). */
 	uint16_t *Wb, *Wtb, *Hb, *Htb;
 	uint32_t ldtW, ldtH;
+	/* Streamed jobs (nmfk_job_create_streamed): V stays in host memory and passes through
+	 * two device slots one row block at a time; V and Vt above stay NULL. */
+	int streamed;
+	const float *hV;               /* host V, row-major, leading dimension hldv */
+	size_t hldv;
+	int hV_pinned;                 /* hV is pinned (nmfk_host_alloc): copied without staging */
+	uint32_t brows, nblocks;       /* rows per block (multiple of NMFK_FIN_ROWS), blocks */
+	float *Vb[2], *Vtb[2];         /* device slots: a row block (ld ldv) and its transpose */
+	size_t ldvtb;
+	float *stage[2];               /* pinned staging buffers for a pageable hV */
+	nmfk_hplan bplanH[2], bplanW[2]; /* plans of a full block [0] and of the last block [1] */
+	float *partH;                  /* the H half's partial sums of all blocks, in block order */
+	uint32_t SH;                   /* slices in partH */
+	float *Hc;                     /* clamped copy of H that test passes contract against */
+	float *den_scratch;            /* where the block partials' denominator tile writes */
+	hipStream_t cstream;           /* host-to-device copies */
+	hipEvent_t up[2], done[2];     /* slot s: block uploaded / block's kernels finished */
+	double stream_bytes;           /* bytes of V copied to the device by the last run */
 };
+
+/* Streamed jobs: the loop and the Frobenius/KL pass (nmfk_stream.c). */
+int nmfk_stream_run(nmfk_job *j, const nmfk_options *opt, nmfk_result *res);
+int nmfk_stream_dot_kl(nmfk_job *j, double *dot_V, double *dot_VWH, double *kl);
+void nmfk_stream_free(nmfk_job *j);
 
 /* Refreshes the BF16 copies of the factor the next half-step reads as B, if its plan is BF16. */
 int nmfk_job_shadow_for_H(nmfk_job *j);

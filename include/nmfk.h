@@ -117,6 +117,26 @@ int nmfk_tune_get(int key);
  * its V must not change while the job uses it (nmfk_job_set_V* on the new job fail). */
 nmfk_job *nmfk_job_create_shared(nmfk_job *src, uint32_t k);
 
+/* A job whose V stays in host memory: the out-of-core counterpart of the reference's
+ * blockwise mode for inputs that do not fit the device (src/GPU_setup.c:902-1270, block
+ * loops of src/NMF_routines.c:543-1000).  V (n x m, row-major, leading dimension ldv) is
+ * copied to the device in row blocks of <block_rows> rows (a multiple of 64; 0 = as many
+ * as fit in 1 GiB of slots) through two slots, once per iteration: each block serves the W
+ * half-step of its rows and then its share of the next H half-step's sums.  pinned != 0
+ * declares that V lies in memory from nmfk_host_alloc (copied directly); otherwise it is
+ * staged through pinned buffers.  V is read, never written, and must outlive the job.
+ * Runs follow the reference's update order and stop rule like nmfk_job_run, in FP32 (kernel
+ * paths VALU or MFMA32); W, H, the class vector and nmfk_job_dot_kl work as for any job,
+ * while the V accessors, the drop-in half-steps, batching and timing refuse such a job. */
+nmfk_job *nmfk_job_create_streamed(uint32_t n, uint32_t m, uint32_t k, const float *V, size_t ldv,
+				   uint32_t block_rows, int pinned);
+/* Rows per block, number of blocks, and bytes of V copied to the device by the last run. */
+int nmfk_job_streamed_info(const nmfk_job *job, uint32_t *block_rows, uint32_t *nblocks, double *bytes_copied);
+
+/* Pinned host memory for a streamed job's V (hipHostMalloc), and its release. */
+void *nmfk_host_alloc(size_t bytes);
+void nmfk_host_free(void *p);
+
 /* Re-plans both half-steps aiming for <target_wgs> partial-kernel workgroups (0 = the
  * process tunables); the number of splits of the contraction range, and with it the
  * summation order, follows the plan, so jobs compared bit for bit must share plans. */

@@ -17,6 +17,8 @@
 #include "nmfk_io.h"
 #include "real_type.h"
 
+#include <hip/hip_runtime_api.h>
+
 #include <inttypes.h>
 #include <math.h>
 #include <stdio.h>
@@ -150,21 +152,41 @@ int main(int argc, char *argv[])
 	float *W = (float *)malloc((size_t)n * k * sizeof(float));
 	float *H = (float *)malloc((size_t)m * k * sizeof(float));
 	uint32_t *cls = (uint32_t *)malloc((size_t)m * sizeof(uint32_t));
-	nmfk_job *job = (W && H && cls) ? nmfk_job_create(n, m, k) : NULL;
+	/* A V that does not fit the device is streamed from host memory in row blocks, as the
+	 * reference switches to blockwise processing (src/GPU_setup.c:902-1270).
+	 * NMFGPU_STREAM_ROWS (environment) forces streaming with that many rows per block (0 =
+	 * the library's default). */
+	const char *env_rows = getenv("NMFGPU_STREAM_ROWS");
+	bool streamed = env_rows && *env_rows;
+	uint32_t stream_rows = streamed ? (uint32_t)strtoul(env_rows, NULL, 10) : 0;
+	if (!streamed) {
+		size_t free_b = 0, total_b = 0;
+		if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && nmfk_job_bytes(n, m, k) > free_b / 10 * 9)
+			streamed = true;
+	}
+	nmfk_job *job = NULL;
+	if (W && H && cls)
+		job = streamed ? nmfk_job_create_streamed(n, m, k, V, m, stream_rows, 0) : nmfk_job_create(n, m, k);
+	if (job && streamed) {
+		uint32_t br = 0, nb = 0;
+		nmfk_job_streamed_info(job, &br, &nb, NULL);
+		print_message(false, "Input matrix streamed from host memory in %" PRIu32 " blocks of %" PRIu32
+				     " rows.\n", nb, br);
+	}
 	if (!job) {
 		print_error(true, "Error setting up the factorization (N=%" PRIu32 ", M=%" PRIu32 ", K=%" PRIu32 "): %s\n", n,
 			    m, k, (W && H && cls) ? nmfk_last_error() : "out of host memory");
 		goto done;
 	}
-	/* the reference's precision: FP32 arithmetic at every rank */
-	if (nmfk_job_kernel_path(job) == 3)
+	/* the reference's precision: FP32 arithmetic at every rank (streamed jobs always are) */
+	if (!streamed && nmfk_job_kernel_path(job) == 3)
 		nmfk_job_set_kernel_path(job, 2);
 	/* NMFGPU_SEED (environment) overrides the seed, for reproducible runs. */
 	index_t seed = get_seed();
 	const char *env_seed = getenv("NMFGPU_SEED");
 	if (env_seed && *env_seed)
 		seed = (index_t)strtoul(env_seed, NULL, 10);
-	if (nmfk_job_set_V(job, V, m) || nmfk_job_init_random(job, seed)) {
+	if ((!streamed && nmfk_job_set_V(job, V, m)) || nmfk_job_init_random(job, seed)) {
 		print_error(true, "Error initializing the factorization: %s\n", nmfk_last_error());
 		goto done;
 	}

@@ -16,10 +16,10 @@ pytestmark = pytest.mark.gpu
 DRIVER = ROOT / "bionmf-gpu_amd" / "NMF_GPU"
 
 
-def run_driver(args, seed=5):
+def run_driver(args, seed=5, **env_extra):
     if not DRIVER.exists():
         pytest.fail("bionmf-gpu_amd/NMF_GPU is not built")
-    env = dict(os.environ, NMFGPU_SEED=str(seed))
+    env = dict(os.environ, NMFGPU_SEED=str(seed), **env_extra)
     return subprocess.run([str(DRIVER)] + [str(a) for a in args], capture_output=True, text=True, env=env,
                           timeout=120)
 
@@ -77,3 +77,26 @@ def test_driver_binary_io_and_errors(nmfk, orc, tmp_path):
     assert r.returncode != 0 and "invalid factorization rank" in r.stderr
     r = run_driver([tmp_path / "missing.txt"])
     assert r.returncode != 0
+
+
+def test_driver_streams_V_from_host_memory(nmfk, orc, tmp_path):
+    # NMFGPU_STREAM_ROWS forces the out-of-core mode the driver picks when V does not fit the
+    # device: same stop iteration and classes as the resident run, factors within 1e-3.
+    n, m, k = 1000, 60, 4
+    V = orc.gen_planted(n, m, 4, 23, 0.3)
+    outs = {}
+    for mode, env in (("resident", {}), ("streamed", {"NMFGPU_STREAM_ROWS": "192"})):
+        d = tmp_path / mode
+        d.mkdir()
+        f = d / "v.txt"
+        nmfk.save_matrix(f, V, nmfk.FMT_TEXT)
+        r = run_driver([f, "-k", k, "-i", 800, "-j", 10, "-t", 30], **env)
+        assert r.returncode == 0, r.stderr
+        assert ("streamed from host memory in 6 blocks of 192 rows" in r.stdout) == (mode == "streamed")
+        it_line = [ln for ln in r.stdout.splitlines() if ln.startswith("NMF:")]
+        outs[mode] = (it_line, nmfk.load_matrix(d / "v.txt_W.txt")[0], nmfk.load_matrix(d / "v.txt_H.txt")[0],
+                      (d / "v.txt_classes.txt").read_text())
+    a, b = outs["resident"], outs["streamed"]
+    assert a[0] == b[0] and a[3] == b[3]
+    assert np.allclose(b[1], a[1], rtol=1e-3, atol=1e-6 * a[1].max())
+    assert np.allclose(b[2], a[2], rtol=1e-3, atol=1e-6 * a[2].max())

@@ -181,6 +181,7 @@ def load(path: os.PathLike | str | None = None):
     _sig(lib, "nmfk_job_plan_get", i32, [vp, _U32P])
     _sig(lib, "nmfk_job_create_streamed", vp, [u32, u32, u32, _F32P, sz, u32, i32])
     _sig(lib, "nmfk_job_streamed_info", i32, [vp, C.POINTER(u32), C.POINTER(u32), C.POINTER(dbl)])
+    _sig(lib, "nmfk_streamed_bytes", sz, [u32, u32, u32, u32])
     _sig(lib, "nmfk_host_alloc", vp, [sz])
     _sig(lib, "nmfk_host_free", None, [vp])
     _sig(lib, "nmfk_batch_default_tuning", None, [C.POINTER(BatchTuning)])

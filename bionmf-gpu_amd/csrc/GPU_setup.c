@@ -112,13 +112,37 @@ int setup_GPU(size_t mem_size, bool do_classf)
 				  ", M=%" PRI_IDX ").\n", K, N, M);
 		return EXIT_FAILURE;
 	}
+	/* When V does not fit in <mem_size>, the input is processed in blocks of rows streamed
+	 * from the host matrix Vrow (pitch Mp), as the reference's setup_GPU() switches to
+	 * blockwise mode (src/GPU_setup.c:902-1270): the largest block (halving from N rows) whose
+	 * buffers fit.  NMFGPU_STREAM_ROWS (environment) forces this mode with that many rows per
+	 * block. */
 	const size_t need = nmfk_job_bytes(N, M, K);
-	if (mem_size && need > mem_size) {
-		print_error(true, "Error: not enough device memory: %zu bytes required, %zu available.\n", need,
-			    mem_size);
-		return EXIT_FAILURE;
+	const char *env_rows = getenv("NMFGPU_STREAM_ROWS");
+	const bool forced = env_rows && *env_rows;
+	const bool streamed = forced || (mem_size && need > mem_size);
+	if (streamed) {
+		const index_t ldv = (M + 3u) & ~3u;
+		if (!Vrow || Mp != ldv) {
+			print_error(true, "Error: blockwise processing needs the host input matrix Vrow with pitch Mp=%" PRI_IDX
+					  " (set: %s, Mp=%" PRI_IDX ").\n", ldv, Vrow ? "yes" : "no", Mp);
+			return EXIT_FAILURE;
+		}
+		uint32_t rows = forced ? (uint32_t)strtoul(env_rows, NULL, 10) : 0;
+		if (!forced) {
+			rows = (N + NMFK_FIN_ROWS - 1) / NMFK_FIN_ROWS * NMFK_FIN_ROWS;
+			while (rows > NMFK_FIN_ROWS && nmfk_streamed_bytes(N, M, K, rows) > mem_size)
+				rows = (rows / 2 + NMFK_FIN_ROWS - 1) / NMFK_FIN_ROWS * NMFK_FIN_ROWS;
+			if (nmfk_streamed_bytes(N, M, K, rows) > mem_size) {
+				print_error(true, "Error: not enough device memory: %zu bytes required for blocks of %u rows, "
+						  "%zu available.\n", nmfk_streamed_bytes(N, M, K, rows), rows, mem_size);
+				return EXIT_FAILURE;
+			}
+		}
+		nmfk_dropin_job = nmfk_job_create_streamed(N, M, K, Vrow, Mp, rows, 1 /* getHostMemory is pinned */);
+	} else {
+		nmfk_dropin_job = nmfk_job_create(N, M, K);
 	}
-	nmfk_dropin_job = nmfk_job_create(N, M, K);
 	if (!nmfk_dropin_job) {
 		print_error(true, "Error: %s\n", nmfk_last_error());
 		return EXIT_FAILURE;
@@ -126,7 +150,7 @@ int setup_GPU(size_t mem_size, bool do_classf)
 	nmfk_job *j = nmfk_dropin_job;
 	/* The reference computes in FP32 throughout: the drop-in keeps every rank on FP32
 	 * arithmetic (BF16 contractions are opt-in through the job API). */
-	if (j->planH.path == NMFK_PATH_MFMA_BF16 && nmfk_job_set_kernel_path(j, NMFK_PATH_MFMA32)) {
+	if (!streamed && j->planH.path == NMFK_PATH_MFMA_BF16 && nmfk_job_set_kernel_path(j, NMFK_PATH_MFMA32)) {
 		print_error(true, "Error: %s\n", nmfk_last_error());
 		return EXIT_FAILURE;
 	}
@@ -134,8 +158,10 @@ int setup_GPU(size_t mem_size, bool do_classf)
 	Mp = (index_t)j->ldv;
 	MpPp = Mp;
 	Kp = j->kt;
-	d_Vrow = j->V;
-	d_Vcol = d_Vrow; /* one resident copy, as in the reference when V fits */
+	/* Streamed: the caller's first upload of block_N.BL[0] rows lands in slot 0 (the job
+	 * copies every block itself); resident: one copy of V, as in the reference when V fits. */
+	d_Vrow = streamed ? j->Vb[0] : j->V;
+	d_Vcol = d_Vrow;
 	d_W = j->W;
 	d_H = j->H;
 	d_WH = NULL;
@@ -151,7 +177,14 @@ int setup_GPU(size_t mem_size, bool do_classf)
 	num_streams_NMF = 1;
 	memset(&block_N, 0, sizeof block_N);
 	memset(&block_M, 0, sizeof block_M);
-	block_N.BL[0] = N, block_N.BLp[0] = N, block_N.num_steps[0] = 1;
+	if (streamed && j->nblocks > 1) {
+		block_N.BL[0] = block_N.BLp[0] = j->brows, block_N.num_steps[0] = N / j->brows;
+		block_N.BL[1] = block_N.BLp[1] = N % j->brows, block_N.num_steps[1] = (N % j->brows) != 0;
+		print_message(false, "Input matrix processed in blocks of %" PRIu32 " rows (%" PRIu32 " blocks).\n", j->brows,
+			      j->nblocks);
+	} else {
+		block_N.BL[0] = N, block_N.BLp[0] = N, block_N.num_steps[0] = 1;
+	}
 	block_M.BL[0] = M, block_M.BLp[0] = Mp, block_M.num_steps[0] = 1;
 	if (create_event(&event_Vrow) || create_event(&event_Vcol) || create_event(&event_W) || create_event(&event_H) ||
 	    create_event(&event_reduction)) {

@@ -161,13 +161,13 @@ static int finish_V(nmfk_job *j)
 
 int nmfk_job_mark_V_changed(nmfk_job *j)
 {
-	j->vt_stale = 1;
+	j->vt_stale = !j->streamed;   /* a streamed job has no device V or Vt */
 	return 0;
 }
 
 int nmfk_job_refresh_Vt(nmfk_job *j)
 {
-	if (!j->vt_stale)
+	if (!j->vt_stale || j->streamed)
 		return 0;
 	j->vt_stale = 0;
 	return nmfk_hip_fail(nmfk_launch_transpose(j->V, j->n, j->m, j->ldv, j->Vt, j->ldvt, j->stream),
@@ -572,8 +572,8 @@ int nmfk_job_shadow_for_W(nmfk_job *j)
 /* update_H(): accum_W = colsum(W) (src/NMF_routines.c:720), then the H half-step. */
 int nmfk_job_update_H(nmfk_job *j)
 {
-	if (needs_device_V(j, "nmfk_job_update_H"))
-		return NMFK_EINVAL;
+	if (j->streamed)
+		return nmfk_stream_update_H(j);
 	int e = nmfk_launch_colsum_parts(j->W, j->n, j->kt, j->colW, j->stream);
 	if (!e)
 		e = nmfk_job_shadow_for_H(j);
@@ -585,8 +585,8 @@ int nmfk_job_update_H(nmfk_job *j)
 /* update_W(): accum_H = colsum(H) (src/NMF_routines.c:1360), then the W half-step. */
 int nmfk_job_update_W(nmfk_job *j)
 {
-	if (needs_device_V(j, "nmfk_job_update_W"))
-		return NMFK_EINVAL;
+	if (j->streamed)
+		return nmfk_stream_update_W(j);
 	if (nmfk_job_refresh_Vt(j))
 		return NMFK_EHIP;
 	int e = nmfk_launch_colsum_parts(j->H, j->m, j->kt, j->colH, j->stream);

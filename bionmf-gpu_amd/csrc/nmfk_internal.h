@@ -65,6 +65,8 @@ struct nmfk_job {
 /* Streamed jobs: the loop and the Frobenius/KL pass (nmfk_stream.c). */
 int nmfk_stream_run(nmfk_job *j, const nmfk_options *opt, nmfk_result *res);
 int nmfk_stream_dot_kl(nmfk_job *j, double *dot_V, double *dot_VWH, double *kl);
+int nmfk_stream_update_H(nmfk_job *j);
+int nmfk_stream_update_W(nmfk_job *j);
 void nmfk_stream_free(nmfk_job *j);
 
 /* Refreshes the BF16 copies of the factor the next half-step reads as B, if its plan is BF16. */

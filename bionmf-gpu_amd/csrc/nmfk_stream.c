@@ -82,6 +82,58 @@ static int stream_path(uint32_t kt)
 	return kt % 16 == 0 ? NMFK_PATH_MFMA32 : NMFK_PATH_VALU;
 }
 
+/* Sizes of a streamed job's buffers, shared by create() and nmfk_streamed_bytes(). */
+typedef struct slayout {
+	uint32_t kt, br, nblocks, rlast, SH, ncolW, ncolH, ndot;
+	size_t ldv, ldvtb, part_cap, bytes;
+	nmfk_hplan bH[2], bW[2];
+} slayout;
+
+static void stream_layout(uint32_t n, uint32_t m, uint32_t k, uint32_t block_rows, slayout *L)
+{
+	L->kt = nmfk_kt(k);
+	L->ldv = rup(m, 4);
+	uint32_t br = block_rows;
+	if (br == 0) {
+		/* two slots of a block and its transpose in 1 GiB */
+		const size_t per_row = 2 * 4 * (L->ldv + m);
+		br = (uint32_t)(((size_t)1 << 30) / per_row / NMFK_FIN_ROWS * NMFK_FIN_ROWS);
+		if (br < NMFK_FIN_ROWS)
+			br = NMFK_FIN_ROWS;
+	}
+	if (br > rup(n, NMFK_FIN_ROWS))
+		br = (uint32_t)rup(n, NMFK_FIN_ROWS);
+	L->br = br;
+	L->nblocks = (n + br - 1) / br;
+	L->rlast = n - (L->nblocks - 1) * br;
+	L->ldvtb = rup(br, 4);
+	const int path = stream_path(L->kt);
+	nmfk_plan_half_path(m, br, L->kt, path, &L->bH[0]);
+	nmfk_plan_half_path(m, L->rlast, L->kt, path, &L->bH[1]);
+	nmfk_plan_half_path(br, m, L->kt, path, &L->bW[0]);
+	nmfk_plan_half_path(L->rlast, m, L->kt, path, &L->bW[1]);
+	L->SH = (L->nblocks - 1) * L->bH[0].S + L->bH[1].S;
+	const size_t pw0 = (size_t)L->bW[0].S * br, pw1 = (size_t)L->bW[1].S * L->rlast;
+	L->part_cap = (pw0 > pw1 ? pw0 : pw1) * L->kt;
+	L->ncolW = (n + NMFK_FIN_ROWS - 1) / NMFK_FIN_ROWS;
+	L->ncolH = (m + NMFK_FIN_ROWS - 1) / NMFK_FIN_ROWS;
+	L->ndot = nmfk_dot_kl_blocks(br, m);
+	const size_t kt = L->kt;
+	L->bytes = 4 * ((size_t)n * kt + 2 * (size_t)m * kt + L->part_cap + (size_t)L->SH * m * kt +
+			((size_t)L->ncolW + L->ncolH + 3) * kt + 2 * (size_t)m) +
+		   8 * 3 * (size_t)L->ndot + sizeof(nmfk_state) +
+		   2 * 4 * ((size_t)br * L->ldv + (size_t)m * L->ldvtb) + 20 * 256;
+}
+
+size_t nmfk_streamed_bytes(uint32_t n, uint32_t m, uint32_t k, uint32_t block_rows)
+{
+	if (n < 1 || m < 1 || k < 1 || k > NMFK_KMAX || block_rows % NMFK_FIN_ROWS != 0)
+		return 0;
+	slayout L;
+	stream_layout(n, m, k, block_rows, &L);
+	return L.bytes;
+}
+
 nmfk_job *nmfk_job_create_streamed(uint32_t n, uint32_t m, uint32_t k, const float *V, size_t ldv,
 				   uint32_t block_rows, int pinned)
 {
@@ -104,37 +156,21 @@ nmfk_job *nmfk_job_create_streamed(uint32_t n, uint32_t m, uint32_t k, const flo
 		nmfk_set_error("nmfk_job_create_streamed: out of host memory");
 		return NULL;
 	}
-	j->n = n, j->m = m, j->k = k, j->kt = nmfk_kt(k);
-	j->ldv = rup(m, 4);
+	slayout L;
+	stream_layout(n, m, k, block_rows, &L);
+	j->n = n, j->m = m, j->k = k, j->kt = L.kt;
+	j->ldv = L.ldv;
 	j->streamed = 1;
 	j->hV = V, j->hldv = ldv, j->hV_pinned = pinned != 0;
-	uint32_t br = block_rows;
-	if (br == 0) {
-		/* two slots of a block and its transpose in 1 GiB */
-		const size_t per_row = 2 * 4 * (j->ldv + m);
-		br = (uint32_t)(((size_t)1 << 30) / per_row / NMFK_FIN_ROWS * NMFK_FIN_ROWS);
-		if (br < NMFK_FIN_ROWS)
-			br = NMFK_FIN_ROWS;
-	}
-	if (br > rup(n, NMFK_FIN_ROWS))
-		br = (uint32_t)rup(n, NMFK_FIN_ROWS);
-	j->brows = br;
-	j->nblocks = (n + br - 1) / br;
-	const uint32_t rlast = n - (j->nblocks - 1) * br;
-	j->ldvtb = rup(br, 4);
-	const int path = stream_path(j->kt);
-	nmfk_plan_half_path(m, br, j->kt, path, &j->bplanH[0]);
-	nmfk_plan_half_path(m, rlast, j->kt, path, &j->bplanH[1]);
-	nmfk_plan_half_path(br, m, j->kt, path, &j->bplanW[0]);
-	nmfk_plan_half_path(rlast, m, j->kt, path, &j->bplanW[1]);
+	j->brows = L.br, j->nblocks = L.nblocks, j->ldvtb = L.ldvtb;
+	const uint32_t br = L.br;
+	for (int i = 0; i < 2; i++)
+		j->bplanH[i] = L.bH[i], j->bplanW[i] = L.bW[i];
 	j->planH = j->bplanH[0];
 	j->planW = j->bplanW[0];
-	j->SH = (j->nblocks - 1) * j->bplanH[0].S + j->bplanH[1].S;
-	size_t pw0 = (size_t)j->bplanW[0].S * br, pw1 = (size_t)j->bplanW[1].S * rlast;
-	j->part_cap = (pw0 > pw1 ? pw0 : pw1) * j->kt;
-	j->ncolW = (n + NMFK_FIN_ROWS - 1) / NMFK_FIN_ROWS;
-	j->ncolH = (m + NMFK_FIN_ROWS - 1) / NMFK_FIN_ROWS;
-	j->ndot = nmfk_dot_kl_blocks(br, m);
+	j->SH = L.SH;
+	j->part_cap = L.part_cap;
+	j->ncolW = L.ncolW, j->ncolH = L.ncolH, j->ndot = L.ndot;
 	const size_t kt = j->kt;
 	int e = 0;
 	e |= salloc((void **)&j->W, 4 * (size_t)n * kt, "hipMalloc(W)");
@@ -171,6 +207,7 @@ nmfk_job *nmfk_job_create_streamed(uint32_t n, uint32_t m, uint32_t k, const flo
 		e |= hipMemsetAsync(j->W, 0, 4 * (size_t)n * kt, j->stream);
 		e |= hipMemsetAsync(j->H, 0, 4 * (size_t)m * kt, j->stream);
 		e |= hipMemsetAsync(j->Hc, 0, 4 * (size_t)m * kt, j->stream);
+		e |= hipMemsetAsync(j->st, 0, sizeof(nmfk_state), j->stream);
 		for (int s = 0; s < 2; s++) {
 			e |= hipMemsetAsync(j->Vb[s], 0, 4 * (size_t)br * j->ldv, j->stream);
 			e |= hipMemsetAsync(j->Vtb[s], 0, 4 * (size_t)m * j->ldvtb, j->stream);
@@ -238,8 +275,9 @@ static int upload(nmfk_job *j, uint32_t b, int s, uint32_t *i0, uint32_t *r)
 /* One pass over the blocks of V.  PASS_H0: the H partial sums against the current W.
  * PASS_WH: update the block's rows of W (clamped if <clamp>), then the H partial sums
  * against the new rows and <Hp>.  PASS_W: the W update only.  PASS_DOT: the Frobenius / KL
- * partials of every block, summed into acc[3] in block order. */
-static int pass(nmfk_job *j, int kind, const float *Hp, int clamp, double acc[3])
+ * partials of every block, summed into acc[3] in block order.  st: the stop state the
+ * kernels skip their work on (null: none). */
+static int pass(nmfk_job *j, int kind, const float *Hp, int clamp, double acc[3], const nmfk_state *st)
 {
 	uint32_t hoff = 0;
 	double *h = NULL;
@@ -250,7 +288,11 @@ static int pass(nmfk_job *j, int kind, const float *Hp, int clamp, double acc[3]
 			return NMFK_ENOMEM;
 		}
 	}
-	int e = 0;
+	/* copies into the slots start after the job stream's earlier work (a caller's own
+	 * upload into slot 0 included) */
+	int e = hipEventRecord(j->ev[0], j->stream);
+	if (!e)
+		e = hipStreamWaitEvent(j->cstream, j->ev[0], 0);
 	for (uint32_t b = 0; b < j->nblocks && !e; b++) {
 		const int s = (int)(b & 1u), lastb = b + 1 == j->nblocks;
 		uint32_t i0, r;
@@ -261,16 +303,16 @@ static int pass(nmfk_job *j, int kind, const float *Hp, int clamp, double acc[3]
 			e = nmfk_launch_transpose(j->Vb[s], r, j->m, j->ldv, j->Vtb[s], j->ldvtb, j->stream);
 			if (!e)
 				e = nmfk_launch_partial(j->Vtb[s], j->ldvtb, Wb, j->H, r, j->m, j->kt, pw, j->part, j->colH,
-							j->ncolH, j->denH, j->st, j->stream);
+							j->ncolH, j->denH, st, j->stream);
 			if (!e)
 				e = nmfk_launch_finish(j->part, pw->S, Wb, r, j->k, j->kt, j->denH,
-						       j->colW + (size_t)(i0 / NMFK_FIN_ROWS) * j->kt, clamp, j->st, j->stream);
+						       j->colW + (size_t)(i0 / NMFK_FIN_ROWS) * j->kt, clamp, st, j->stream);
 		}
 		if (!e && (kind == PASS_H0 || kind == PASS_WH)) {
 			/* the partial kernel's denominator tile reduces no parts into a scratch buffer:
 			 * colsum(W) is complete only after the pass */
 			e = nmfk_launch_partial(j->Vb[s], j->ldv, Hp, Wb, j->m, r, j->kt, ph,
-						j->partH + (size_t)hoff * j->m * j->kt, j->colW, 0, j->den_scratch, j->st,
+						j->partH + (size_t)hoff * j->m * j->kt, j->colW, 0, j->den_scratch, st,
 						j->stream);
 			hoff += ph->S;
 		}
@@ -307,7 +349,7 @@ int nmfk_stream_run(nmfk_job *j, const nmfk_options *opt, nmfk_result *res)
 	const double t0 = nmfk_now();
 	if (opt->nIters > 0) {
 		/* iteration 0's H half-step */
-		e = pass(j, PASS_H0, j->H, 0, NULL);
+		e = pass(j, PASS_H0, j->H, 0, NULL, j->st);
 		if (!e)
 			e = nmfk_hip_fail(nmfk_launch_finish(j->partH, j->SH, j->H, j->m, j->k, j->kt, j->denW, j->colH, 0,
 							     j->st, j->stream),
@@ -327,7 +369,7 @@ int nmfk_stream_run(nmfk_job *j, const nmfk_options *opt, nmfk_result *res)
 			}
 			Hp = j->Hc;
 		}
-		e = pass(j, more ? PASS_WH : PASS_W, Hp, test, NULL);
+		e = pass(j, more ? PASS_WH : PASS_W, Hp, test, NULL, j->st);
 		if (!e && test)
 			e = nmfk_hip_fail(nmfk_launch_test(j->H, j->m, j->k, j->kt, j->cls, j->last, j->st, j->stream),
 					  "test");
@@ -373,7 +415,7 @@ int nmfk_stream_dot_kl(nmfk_job *j, double *dot_V, double *dot_VWH, double *kl)
 {
 	double acc[3] = {0.0, 0.0, 0.0};
 	const double run_bytes = j->stream_bytes;   /* reported for runs only */
-	int e = pass(j, PASS_DOT, NULL, 0, acc);
+	int e = pass(j, PASS_DOT, NULL, 0, acc, NULL);
 	j->stream_bytes = run_bytes;
 	if (e)
 		return e;
@@ -384,4 +426,31 @@ int nmfk_stream_dot_kl(nmfk_job *j, double *dot_V, double *dot_VWH, double *kl)
 	if (kl)
 		*kl = acc[2];
 	return 0;
+}
+
+/* update_H() of a streamed job (the reference's blockwise half-step): colsum(W), one pass
+ * over V adding the H partial sums, then the H finish.  Not gated by the stop state. */
+int nmfk_stream_update_H(nmfk_job *j)
+{
+	int e = nmfk_launch_colsum_parts(j->W, j->n, j->kt, j->colW, j->stream);
+	if (!e)
+		e = nmfk_launch_den(j->colW, j->ncolW, j->kt, j->denW, j->stream);
+	if (e)
+		return nmfk_hip_fail(e, "update_H");
+	e = pass(j, PASS_H0, j->H, 0, NULL, NULL);
+	if (!e)
+		e = nmfk_hip_fail(nmfk_launch_finish(j->partH, j->SH, j->H, j->m, j->k, j->kt, j->denW, j->colH, 0, NULL,
+						     j->stream),
+				  "update_H");
+	return e;
+}
+
+/* update_W() of a streamed job: colsum(H), then one pass over V updating the rows of W
+ * block by block. */
+int nmfk_stream_update_W(nmfk_job *j)
+{
+	int e = nmfk_launch_colsum_parts(j->H, j->m, j->kt, j->colH, j->stream);
+	if (e)
+		return nmfk_hip_fail(e, "update_W");
+	return pass(j, PASS_W, NULL, 0, NULL, NULL);
 }

@@ -127,9 +127,12 @@ nmfk_job *nmfk_job_create_shared(nmfk_job *src, uint32_t k);
  * staged through pinned buffers.  V is read, never written, and must outlive the job.
  * Runs follow the reference's update order and stop rule like nmfk_job_run, in FP32 (kernel
  * paths VALU or MFMA32); W, H, the class vector and nmfk_job_dot_kl work as for any job,
- * while the V accessors, the drop-in half-steps, batching and timing refuse such a job. */
+ * and so do nmfk_job_update_H / nmfk_job_update_W (one pass over V each, as the reference's
+ * blockwise half-steps), while the V accessors, batching and timing refuse such a job. */
 nmfk_job *nmfk_job_create_streamed(uint32_t n, uint32_t m, uint32_t k, const float *V, size_t ldv,
 				   uint32_t block_rows, int pinned);
+/* Device bytes nmfk_job_create_streamed allocates for these arguments (0: invalid). */
+size_t nmfk_streamed_bytes(uint32_t n, uint32_t m, uint32_t k, uint32_t block_rows);
 /* Rows per block, number of blocks, and bytes of V copied to the device by the last run. */
 int nmfk_job_streamed_info(const nmfk_job *job, uint32_t *block_rows, uint32_t *nblocks, double *bytes_copied);
 

@@ -57,14 +57,41 @@ def gp(lib, name):
     return C.c_void_p.in_dll(lib, name).value
 
 
-def reference_style_nmf(lib, V, k, seed, nIters, T, S):
-    """src/NMF_GPU.c:312-795 written against the drop-in API (no blockwise plan)."""
+class BlockT(C.Structure):
+    """block_t of GPU_setup.h: rows or columns of V per block on the device."""
+    _fields_ = [("BL", u32 * 2), ("BLp", u32 * 2), ("num_steps", u32 * 2)]
+
+
+def reference_style_nmf(lib, V, k, seed, nIters, T, S, mem_size=None, nmfk=None):
+    """src/NMF_GPU.c:312-795 written against the drop-in API.  With <mem_size>, the host
+    matrix Vrow (pinned, pitch Mp) is set before setup_GPU as init_V() does
+    (src/NMF_GPU.c:212-303), and the device budget decides the blockwise plan."""
     n, m = V.shape
     free_mem = lib.initialize_GPU(0, k)
     assert free_mem > 0
     g(lib, "N").value, g(lib, "M").value = n, m
-    assert lib.setup_GPU(free_mem, True) == 0
+    host = None
+    if mem_size is not None:
+        Mp = (m + 3) // 4 * 4
+        host = nmfk.PinnedArray((n, Mp))
+        host.array[...] = 0
+        host.array[:, :m] = V
+        g(lib, "Mp").value = Mp
+        C.c_void_p.in_dll(lib, "Vrow").value = host.ptr
+    try:
+        return _reference_style_loop(lib, V, k, seed, nIters, T, S, free_mem if mem_size is None else mem_size,
+                                     host)
+    finally:
+        if host is not None:
+            C.c_void_p.in_dll(lib, "Vrow").value = None
+            host.close()
+
+
+def _reference_style_loop(lib, V, k, seed, nIters, T, S, mem_size, host):
+    n, m = V.shape
+    assert lib.setup_GPU(mem_size, True) == 0
     Kp, Mp = g(lib, "Kp").value, g(lib, "Mp").value
+    BLN = BlockT.in_dll(lib, "block_N").BL[0]
     stream_W, stream_H = gp(lib, "stream_W"), gp(lib, "stream_H")
     ev_W = C.addressof(C.c_void_p.in_dll(lib, "event_W"))
     try:
@@ -72,10 +99,14 @@ def reference_style_nmf(lib, V, k, seed, nIters, T, S):
         assert lib.set_random_values(gp(lib, "d_W"), n, k, Kp, b"d_W", stream_W, ev_W) == 0
         assert lib.set_random_values(gp(lib, "d_H"), m, k, Kp, b"d_H", stream_H, None) == 0
         assert lib.destroy_random() == 0
-        Vrow = np.zeros((n, Mp), np.float32)
-        Vrow[:, :m] = V
+        if host is None:
+            Vrow = np.zeros((n, Mp), np.float32)
+            Vrow[:, :m] = V
+        else:
+            Vrow = host.array
         ev_Vrow = C.c_void_p.in_dll(lib, "event_Vrow").value
-        assert lib.upload_matrix_partial(Vrow, n, Mp, 0, 0, b"Vrow", Mp, gp(lib, "d_Vrow"),
+        # the driver uploads the first block of rows (src/NMF_GPU.c:494)
+        assert lib.upload_matrix_partial(Vrow, BLN, Mp, 0, 0, b"Vrow", Mp, gp(lib, "d_Vrow"),
                                          gp(lib, "stream_Vrow"), ev_Vrow) == 0
         cls = np.zeros(m, np.uint32)
         last = np.zeros(m, np.uint32)
@@ -111,7 +142,7 @@ def reference_style_nmf(lib, V, k, seed, nIters, T, S):
         dv, dvwh = C.c_float(), C.c_float()
         assert lib.dot_product_VWH(C.byref(dv), C.byref(dvwh)) == 0
         return {"W": W[:, :k], "H": H[:, :k], "iterations": iters, "converged": converged,
-                "cls": last.copy(), "dist": float(np.sqrt(dvwh.value) / np.sqrt(dv.value))}
+                "cls": last.copy(), "dist": float(np.sqrt(dvwh.value) / np.sqrt(dv.value)), "BLN": BLN}
     finally:
         assert lib.finalize_GPU_device() == 0
 
@@ -133,6 +164,27 @@ def test_reference_style_driver_matches_oracle_and_job_api(lib, nmfk, orc, n, m,
         jr = j.run(500, 10, 6)
         assert jr["iterations"] == r["iterations"]
         assert np.array_equal(j.get_H(), r["H"]) and np.array_equal(j.get_W(), r["W"])
+
+
+def test_reference_style_driver_blockwise(lib, nmfk, orc):
+    # A device budget below the resident job's size: setup_GPU() takes the largest block of
+    # rows that fits and update_H/update_W stream the host matrix Vrow, as the reference's
+    # blockwise mode.  Same stop iteration and classes as the FP64 oracle and the resident
+    # drop-in run; factors within 1e-3.
+    n, m, k = 1500, 90, 5
+    V = orc.gen_lognormal(n, m, 32, 0.0, 1.0)
+    budget = nmfk.load().nmfk_streamed_bytes(n, m, k, 256)
+    r = reference_style_nmf(lib, V, k, seed=7, nIters=400, T=10, S=6, mem_size=budget, nmfk=nmfk)
+    assert 64 <= r["BLN"] <= 256
+    res = reference_style_nmf(lib, V, k, seed=7, nIters=400, T=10, S=6)
+    assert res["BLN"] == n
+    W0, H0 = orc.init_WH(n, m, k, 7)
+    o = orc.nmf(V, W0, H0, 400, 10, 6, fp64=True)
+    for x in (r, res):
+        assert x["iterations"] == o["iterations"] and x["converged"] == o["converged"]
+        assert np.array_equal(x["cls"], orc.idx_max(o["H"]))
+    assert rel_err(r["W"], res["W"]) < 1e-3 and rel_err(r["H"], res["H"]) < 1e-3
+    assert abs(r["dist"] - res["dist"]) < 1e-5
 
 
 def test_matrix_operations_match_oracle(lib, orc):

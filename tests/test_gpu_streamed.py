@@ -133,7 +133,7 @@ def test_streamed_job_refuses_device_V_operations(nmfk, orc):
         with pytest.raises(nmfk.NmfkError, match="streamed"):
             j.set_V(V)
         with pytest.raises(nmfk.NmfkError, match="streamed"):
-            j.update_H()
+            j.get_V()
         with pytest.raises(nmfk.NmfkError, match="streamed"):
             j.time_iterations(2)
         with pytest.raises(nmfk.NmfkError, match="streamed"):
@@ -142,3 +142,27 @@ def test_streamed_job_refuses_device_V_operations(nmfk, orc):
         with pytest.raises(nmfk.NmfkError, match="streamed"):
             batch.add(j, 100, 40)
         batch.close()
+
+
+def test_streamed_half_steps_match_resident(nmfk, orc):
+    # nmfk_job_update_H / update_W on a streamed job: one pass over V per half-step, the
+    # reference's blockwise structure, with the resident job's update order and clamps.
+    n, m, k = 1700, 130, 6
+    V = orc.gen_lognormal(n, m, 12, 0.0, 1.0)
+    a, b = resident_and_streamed(nmfk, V, k, 320, 44)
+    for step in range(12):
+        for j in (a, b):
+            j.update_H()
+            j.update_W()
+            if step % 4 == 3:
+                j.adjust()
+    assert rel_err(b.get_W(), a.get_W()) < 1e-5 and rel_err(b.get_H(), a.get_H()) < 1e-5
+    W0, H0 = orc.init_WH(n, m, k, 44)
+    W, H = W0.astype(np.float64), H0.astype(np.float64)
+    for step in range(12):
+        H = orc.update_H(V, W, H)
+        W = orc.update_W(V, W, H)
+        if step % 4 == 3:
+            H, W = orc.adjust(H), orc.adjust(W)
+    assert rel_err(b.get_W(), W) < 1e-4 and rel_err(b.get_H(), H) < 1e-4
+    a.close(), b.close()

@@ -21,12 +21,14 @@
 #include <stdlib.h>
 #include <string.h>
 
-/* Partial tables per half and body class (VALU jobs kt <= 8, matrix-core jobs kt 16/32). */
-enum { TAB_PART_H0, TAB_PART_H1, TAB_PART_W0, TAB_PART_W1, TAB_FIN_H, TAB_FIN_W, TAB_TEST, NTAB };
+/* Partial tables per half and body class: VALU jobs (kt <= 8), matrix-core jobs of kt 16
+ * and of kt 32, each launched on its own so that it runs at its own occupancy. */
+#define NCLS 3
+enum { TAB_PART_H0, TAB_PART_W0 = TAB_PART_H0 + NCLS, TAB_FIN_H = TAB_PART_W0 + NCLS, TAB_FIN_W, TAB_TEST, NTAB };
 
 static int job_cls(const nmfk_job *j)
 {
-	return j->kt <= 8 ? 0 : 1;
+	return j->kt <= 8 ? 0 : j->kt == 16 ? 1 : 2;
 }
 
 struct nmfk_batch {
@@ -290,8 +292,8 @@ static int build_tables(nmfk_batch *b)
 	}
 	uint32_t n = 0;
 	for (int half = 0; half < 2; half++)
-		for (int cls = 0; cls < 2; cls++) {
-			const int tab = TAB_PART_H0 + 2 * half + cls;
+		for (int cls = 0; cls < NCLS; cls++) {
+			const int tab = TAB_PART_H0 + NCLS * half + cls;
 			b->off[tab] = n;
 			push_partial(b, b->h_tiles, &n, half, cls);
 			b->cnt[tab] = n - b->off[tab];
@@ -351,13 +353,13 @@ static int enqueue_iteration(nmfk_batch *b, int clamp)
 {
 	const nmfk_tile *t = b->d_tiles;
 	int e = 0;
-	for (int cls = 0; cls < 2 && !e; cls++)
+	for (int cls = 0; cls < NCLS && !e; cls++)
 		e = nmfk_launch_batch_partial(b->d_desc, t + b->off[TAB_PART_H0 + cls], b->cnt[TAB_PART_H0 + cls], 0,
 					      cls, b->tune.grid_partial, b->stream);
 	if (!e)
 		e = nmfk_launch_batch_finish(b->d_desc, t + b->off[TAB_FIN_H], b->cnt[TAB_FIN_H], 0, 0,
 					     b->tune.grid_finish, b->stream);
-	for (int cls = 0; cls < 2 && !e; cls++)
+	for (int cls = 0; cls < NCLS && !e; cls++)
 		e = nmfk_launch_batch_partial(b->d_desc, t + b->off[TAB_PART_W0 + cls], b->cnt[TAB_PART_W0 + cls], 1,
 					      cls, b->tune.grid_partial, b->stream);
 	if (!e)

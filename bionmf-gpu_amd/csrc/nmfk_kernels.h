@@ -155,7 +155,8 @@ typedef struct nmfk_tile {
 #define NMFK_BATCH_KTMAX 32
 
 /* half: 0 = H half (U = V, A = H, B = W), 1 = W half (U = Vt, A = W, B = H).  cls: the body
- * class of every tile's job, 0 = vector ALUs (kt 4/8), 1 = matrix cores (kt 16/32). */
+ * class of every tile's job, 0 = vector ALUs (kt 4/8), 1 = matrix cores at kt 16, 2 = matrix
+ * cores at kt 32; tiles of other jobs are skipped. */
 int nmfk_launch_batch_partial(const nmfk_bjob *jobs, const nmfk_tile *tiles, uint32_t ntiles, int half,
 			      int cls, uint32_t grid, hipStream_t stream);
 int nmfk_launch_batch_finish(const nmfk_bjob *jobs, const nmfk_tile *tiles, uint32_t ntiles, int half,

@@ -1383,24 +1383,26 @@ __device__ __forceinline__ half_args batch_half(const nmfk_bjob &J, int half)
 	return h;
 }
 
-// Two batched partial kernels, one per body class, so each runs at its own occupancy:
-// VALU bodies (KT 4/8, <= 136 VGPRs) and matrix-core bodies (KT 16/32); a batch launches
-// both (each over its own tile table).
+// Three batched partial kernels, one per body class, so each runs at its own occupancy:
+// VALU bodies (KT 4/8), the matrix-core body of KT 16 and that of KT 32; a batch launches
+// each over its own tile table.
 template <int CLS>
 constexpr int smem_batch_partial()
 {
-	return CLS == 0 ? smem_valu(8, 4, 4) : smem_mfma(NMFK_BATCH_KTMAX, 1, 4);
+	return CLS == 0 ? smem_valu(8, 4, 4) : smem_mfma(CLS == 1 ? 16 : 32, 1, 4);
 }
 
 template <int CLS>
-__global__ __launch_bounds__(NMFK_WG) void k_batch_partial(const nmfk_bjob *__restrict__ jobs,
+__global__ __launch_bounds__(NMFK_WG, (CLS == 1 ? 4 : 1)) void k_batch_partial(const nmfk_bjob *__restrict__ jobs,
 							   const nmfk_tile *__restrict__ tiles, int ntiles, int half)
 {
 	__shared__ float smem[smem_batch_partial<CLS>()];
 	for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
 		const nmfk_tile tl = tiles[t];
 		const nmfk_bjob &J = jobs[tl.job];
-		if (!J.live || J.st->converged)
+		// a job of another class would not fit this kernel's body: never run it here
+		const bool fits = CLS == 0 ? J.kt <= 8 : J.kt == (CLS == 1 ? 16u : 32u);
+		if (!fits || !J.live || J.st->converged)
 			continue;
 		const half_args h = batch_half(J, half);
 		if constexpr (CLS == 0) {
@@ -1408,11 +1410,10 @@ __global__ __launch_bounds__(NMFK_WG) void k_batch_partial(const nmfk_bjob *__re
 				partial_valu_body<4, 4, 4>(tl.bx, tl.by, h, smem);
 			else
 				partial_valu_body<8, 4, 4>(tl.bx, tl.by, h, smem);
+		} else if constexpr (CLS == 1) {
+			partial_mfma_body<16, 1, 4>(tl.bx, tl.by, h, smem);
 		} else {
-			if (J.kt == 16)
-				partial_mfma_body<16, 1, 4>(tl.bx, tl.by, h, smem);
-			else
-				partial_mfma_body<32, 1, 4>(tl.bx, tl.by, h, smem);
+			partial_mfma_body<32, 1, 4>(tl.bx, tl.by, h, smem);
 		}
 		__syncthreads();   // the next tile reuses the LDS
 	}
@@ -2031,9 +2032,14 @@ extern "C" int nmfk_launch_batch_partial(const nmfk_bjob *jobs, const nmfk_tile 
 	if (cls == 0)
 		hipLaunchKernelGGL((k_batch_partial<0>), dim3(batch_grid(ntiles, grid)), dim3(NMFK_WG), 0, stream, jobs,
 				   tiles, (int)ntiles, half);
-	else
+	else if (cls == 1)
 		hipLaunchKernelGGL((k_batch_partial<1>), dim3(batch_grid(ntiles, grid)), dim3(NMFK_WG), 0, stream, jobs,
 				   tiles, (int)ntiles, half);
+	else if (cls == 2)
+		hipLaunchKernelGGL((k_batch_partial<2>), dim3(batch_grid(ntiles, grid)), dim3(NMFK_WG), 0, stream, jobs,
+				   tiles, (int)ntiles, half);
+	else
+		return (int)hipErrorInvalidValue;
 	return (int)hipGetLastError();
 }
 

@@ -165,8 +165,17 @@ int main(int argc, char *argv[])
 			streamed = true;
 	}
 	nmfk_job *job = NULL;
+	/* A streamed V is copied once into pinned memory, which the device reads directly at the
+	 * PCIe rate; if that allocation fails it is staged from the loaded matrix instead. */
+	float *Vpin = NULL;
+	if (streamed && W && H && cls) {
+		Vpin = (float *)nmfk_host_alloc((size_t)n * m * sizeof(float));
+		if (Vpin)
+			memcpy(Vpin, V, (size_t)n * m * sizeof(float));
+	}
 	if (W && H && cls)
-		job = streamed ? nmfk_job_create_streamed(n, m, k, V, m, stream_rows, 0) : nmfk_job_create(n, m, k);
+		job = streamed ? nmfk_job_create_streamed(n, m, k, Vpin ? Vpin : V, m, stream_rows, Vpin != NULL)
+			       : nmfk_job_create(n, m, k);
 	if (job && streamed) {
 		uint32_t br = 0, nb = 0;
 		nmfk_job_streamed_info(job, &br, &nb, NULL);
@@ -214,6 +223,7 @@ int main(int argc, char *argv[])
 		print_message(false, "Done.\n");
 done:
 	nmfk_job_destroy(job);
+	nmfk_host_free(Vpin);
 	free(W), free(H), free(cls), free(V);
 	nmfk_tags_free(&tags);
 	return status;

@@ -475,8 +475,8 @@ __device__ __forceinline__ void partial_mfma_body(int bx, int by, const half_arg
 
 	// Lanes past p read column 0 (in bounds); their rows a only reach MFMA output rows of
 	// the same invalid a, which are never stored, so no column mask is needed.
-	const __amdgpu_buffer_rsrc_t rU = make_rsrc(U, (size_t)q * ldu * 4);
-	const __amdgpu_buffer_rsrc_t rB = make_rsrc(B, (size_t)q * KT * 4);
+	// The buffer descriptors are based at each 16-row block (wave-uniform scalar work), so
+	// byte offsets stay within one block and U and B may exceed the 32-bit range of an offset.
 	const int ucol = avail > 0 ? aj : 0;
 	int uoff[4], b2off[4][H], b1off[H];
 #pragma unroll
@@ -500,17 +500,18 @@ __device__ __forceinline__ void partial_mfma_body(int bx, int by, const half_arg
 		float b2[4][H];
 	};
 	auto load_full = [&](int bb, blk &x) {
-		const int su = bb * (int)ldu * 4, sb = bb * KT * 4;
+		const __amdgpu_buffer_rsrc_t rU = make_rsrc(U + (size_t)bb * ldu, (size_t)16 * ldu * 4);
+		const __amdgpu_buffer_rsrc_t rB = make_rsrc(B + (size_t)bb * KT, (size_t)16 * KT * 4);
 #pragma unroll
 		for (int r = 0; r < 4; r++) {
-			x.u[r] = bload4(rU, uoff[r], su);
+			x.u[r] = bload4(rU, uoff[r], 0);
 #pragma unroll
 			for (int h = 0; h < H; h++)
-				x.b2[r][h] = bload1(rB, b2off[r][h], sb);
+				x.b2[r][h] = bload1(rB, b2off[r][h], 0);
 		}
 #pragma unroll
 		for (int h = 0; h < H; h++)
-			x.b1[h] = bload4(rB, b1off[h], sb);
+			x.b1[h] = bload4(rB, b1off[h], 0);
 	};
 	// Partial last block of the matrix: rows at or past b_end read row b_begin instead.
 	auto load_tail = [&](int bb, blk &x) {
@@ -1836,7 +1837,8 @@ static int launch_partial_mfma_t(const float *U, size_t ldu, const float *A, con
 {
 	const dim3 grid = xcd_grid(pl);
 	const int na = (int)pl->ap, nb = pl->ap ? (int)(pl->nw / pl->ap) : 0;
-	if (pl->bchunk % 16 != 0 || !mfma_shape_ok(na, nb) || (size_t)q * ldu * 4 > 0x7fffffffull)
+	// byte offsets within one 16-row block of U are 32-bit
+	if (pl->bchunk % 16 != 0 || !mfma_shape_ok(na, nb) || (size_t)16 * ldu * 4 > 0x7fffffffull)
 		return (int)hipErrorInvalidValue;
 	const half_args h = {U, ldu, A, B, (int)p, (int)q, (int)pl->bchunk, (int)pl->nab, part, colB, (int)ncolB, den};
 #define NMFK_MFMA_SHAPE(NA_, NB_)                                                                        \

@@ -248,3 +248,21 @@ def test_bf16_path_error_against_fp64(nmfk, orc):
     assert fro < 2e-2
     assert abs(kl - klo) / klo < 1e-2
     assert agree > 0.8
+
+
+@pytest.mark.parametrize("k", [16, 24])
+def test_matrix_core_path_beyond_2GiB(nmfk, orc, k):
+    # V 600000 x 1000 (2.4 GB, and its transposed copy): both half-steps stream a U larger
+    # than a 32-bit byte offset reaches.  The matrix-core path must agree with the
+    # vector-ALU path (plain pointers) on the same start.
+    n, m = 600000, 1000
+    with nmfk.Job(n, m, k) as j:
+        j.gen_V_planted(77, 12, 0.3)
+        out = {}
+        for path in (2, 1):
+            j.set_kernel_path(path)
+            j.init_random(3)
+            r = j.run(6, 3, 1000)
+            assert r["iterations"] == 6
+            out[path] = (j.get_W(), j.get_H())
+    assert rel_err(out[2][0], out[1][0]) < 1e-4 and rel_err(out[2][1], out[1][1]) < 1e-4

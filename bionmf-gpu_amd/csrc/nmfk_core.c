@@ -174,6 +174,62 @@ int nmfk_job_refresh_Vt(nmfk_job *j)
 			     "transpose V");
 }
 
+/* Host <-> device copies of <rows> rows of <width> floats between pitches (in floats).
+ * Unpadded rows on both sides are one contiguous transfer; otherwise the rows are repacked
+ * on the host, so that a tall matrix of narrow rows is never moved row by row.  Device
+ * padding columns are written as zeros and host padding columns are left alone.  Both wait
+ * for the copy (the stream is synchronized). */
+static int h2d_rows(float *d, size_t dpitch, const float *h, size_t hpitch, size_t width, size_t rows,
+		    hipStream_t stream)
+{
+	if (rows == 0)
+		return 0;
+	const size_t bytes = rows * dpitch * 4;
+	float *tmp = NULL;
+	const float *src = h;
+	if (hpitch != width || dpitch != width) {
+		tmp = (float *)calloc(rows * dpitch, 4);
+		if (!tmp) {
+			nmfk_set_error("out of host memory");
+			return NMFK_ENOMEM;
+		}
+		for (size_t i = 0; i < rows; i++)
+			memcpy(tmp + i * dpitch, h + i * hpitch, width * 4);
+		src = tmp;
+	}
+	int e = hipMemcpyAsync(d, src, bytes, hipMemcpyHostToDevice, stream);
+	if (!e)
+		e = hipStreamSynchronize(stream);
+	free(tmp);
+	return nmfk_hip_fail(e, "host-to-device copy");
+}
+
+static int d2h_rows(float *h, size_t hpitch, const float *d, size_t dpitch, size_t width, size_t rows,
+		    hipStream_t stream)
+{
+	if (rows == 0)
+		return 0;
+	const size_t bytes = rows * dpitch * 4;
+	float *tmp = NULL;
+	float *dst = h;
+	if (hpitch != width || dpitch != width) {
+		tmp = (float *)malloc(bytes);
+		if (!tmp) {
+			nmfk_set_error("out of host memory");
+			return NMFK_ENOMEM;
+		}
+		dst = tmp;
+	}
+	int e = hipMemcpyAsync(dst, d, bytes, hipMemcpyDeviceToHost, stream);
+	if (!e)
+		e = hipStreamSynchronize(stream);
+	if (!e && tmp)
+		for (size_t i = 0; i < rows; i++)
+			memcpy(h + i * hpitch, tmp + i * dpitch, width * 4);
+	free(tmp);
+	return nmfk_hip_fail(e, "device-to-host copy");
+}
+
 static nmfk_job *job_create(uint32_t n, uint32_t m, uint32_t k, nmfk_job *src);
 
 nmfk_job *nmfk_job_create(uint32_t n, uint32_t m, uint32_t k)
@@ -386,10 +442,9 @@ int nmfk_job_set_V(nmfk_job *j, const float *V, size_t ldv)
 		nmfk_set_error("nmfk_job_set_V: ldv=%zu < m=%u", ldv, j->m);
 		return NMFK_EINVAL;
 	}
-	int e = hipMemcpy2DAsync(j->V, j->ldv * 4, V, ldv * 4, (size_t)j->m * 4, j->n, hipMemcpyHostToDevice,
-				 j->stream);
+	int e = h2d_rows(j->V, j->ldv, V, ldv, j->m, j->n, j->stream);
 	if (e)
-		return nmfk_hip_fail(e, "upload V");
+		return e;
 	return finish_V(j);
 }
 
@@ -444,11 +499,7 @@ int nmfk_job_get_V(nmfk_job *j, float *V, size_t ldv)
 		nmfk_set_error("nmfk_job_get_V: ldv < m");
 		return NMFK_EINVAL;
 	}
-	int e = hipMemcpy2DAsync(V, ldv * 4, j->V, j->ldv * 4, (size_t)j->m * 4, j->n, hipMemcpyDeviceToHost,
-				 j->stream);
-	if (e)
-		return nmfk_hip_fail(e, "download V");
-	return nmfk_job_sync(j);
+	return d2h_rows(V, ldv, j->V, j->ldv, j->m, j->n, j->stream);
 }
 
 int nmfk_job_init_random(nmfk_job *j, uint64_t seed)
@@ -470,11 +521,8 @@ static int put_factor(nmfk_job *j, float *d, uint32_t rows, const float *h, size
 		nmfk_set_error("leading dimension %zu < k=%u", ld, j->k);
 		return NMFK_EINVAL;
 	}
-	int e = hipMemsetAsync(d, 0, (size_t)rows * j->kt * 4, j->stream);
-	if (!e)
-		e = hipMemcpy2DAsync(d, (size_t)j->kt * 4, h, ld * 4, (size_t)j->k * 4, rows, hipMemcpyHostToDevice,
-				     j->stream);
-	return nmfk_hip_fail(e, "upload factor");
+	/* the padding columns k..kt-1 are written as zeros */
+	return h2d_rows(d, j->kt, h, ld, j->k, rows, j->stream);
 }
 
 int nmfk_job_set_WH(nmfk_job *j, const float *W, size_t ldw, const float *H, size_t ldh)
@@ -494,11 +542,9 @@ static int get_factor(nmfk_job *j, const float *d, uint32_t rows, float *h, size
 		nmfk_set_error("leading dimension %zu < k=%u", ld, j->k);
 		return NMFK_EINVAL;
 	}
-	int e = hipMemcpy2DAsync(h, ld * 4, d, (size_t)j->kt * 4, (size_t)j->k * 4, rows, hipMemcpyDeviceToHost,
-				 j->stream);
-	if (e)
-		return nmfk_hip_fail(e, "download factor");
-	return nmfk_job_sync(j);
+	if (nmfk_job_sync(j))
+		return NMFK_EHIP;
+	return d2h_rows(h, ld, d, j->kt, j->k, rows, j->stream);
 }
 
 int nmfk_job_get_W(nmfk_job *j, float *W, size_t ldw)

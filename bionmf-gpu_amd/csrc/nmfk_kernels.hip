@@ -1581,17 +1581,21 @@ __global__ __launch_bounds__(256) void k_transpose(const float *__restrict__ V, 
 						   float *__restrict__ Vt, size_t ldvt)
 {
 	__shared__ float tile[64][65];
-	const int i0 = blockIdx.y * 64, j0 = blockIdx.x * 64;
+	const int j0 = blockIdx.x * 64;
 	const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-	for (int r = ty; r < 64; r += 4) {
-		const int i = i0 + r, j = j0 + tx;
-		tile[r][tx] = (i < n && j < m) ? V[(size_t)i * ldv + j] : 0.0f;
-	}
-	__syncthreads();
-	for (int r = ty; r < 64; r += 4) {
-		const int j = j0 + r, i = i0 + tx;
-		if (j < m && i < n)
-			Vt[(size_t)j * ldvt + i] = tile[tx][r];
+	// 64-row tiles strided over the grid's y extent (at most 65535 blocks)
+	for (int i0 = blockIdx.y * 64; i0 < n; i0 += gridDim.y * 64) {
+		for (int r = ty; r < 64; r += 4) {
+			const int i = i0 + r, j = j0 + tx;
+			tile[r][tx] = (i < n && j < m) ? V[(size_t)i * ldv + j] : 0.0f;
+		}
+		__syncthreads();
+		for (int r = ty; r < 64; r += 4) {
+			const int j = j0 + r, i = i0 + tx;
+			if (j < m && i < n)
+				Vt[(size_t)j * ldvt + i] = tile[tx][r];
+		}
+		__syncthreads();   // the tile is rewritten by the next row block
 	}
 }
 
@@ -2123,7 +2127,8 @@ extern "C" int nmfk_launch_gen_planted(float *V, uint32_t n, uint32_t m, uint32_
 extern "C" int nmfk_launch_transpose(const float *V, uint32_t n, uint32_t m, size_t ldv, float *Vt, size_t ldvt,
 				     hipStream_t stream)
 {
-	dim3 grid((m + 63) / 64, (n + 63) / 64);
+	const uint32_t rows = (n + 63) / 64;
+	dim3 grid((m + 63) / 64, rows < 65535u ? rows : 65535u);
 	hipLaunchKernelGGL(k_transpose, grid, dim3(256), 0, stream, V, (int)n, (int)m, ldv, Vt, ldvt);
 	return (int)hipGetLastError();
 }

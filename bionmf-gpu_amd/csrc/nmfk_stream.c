@@ -252,7 +252,11 @@ static int upload(nmfk_job *j, uint32_t b, int s, uint32_t *i0, uint32_t *r)
 	const size_t row = 4 * (size_t)j->m;
 	int e = hipStreamWaitEvent(j->cstream, j->done[s], 0);
 	if (!e && j->hV_pinned) {
-		e = hipMemcpy2DAsync(j->Vb[s], j->ldv * 4, src, j->hldv * 4, row, *r, hipMemcpyHostToDevice, j->cstream);
+		if (j->hldv == j->m && j->ldv == j->m)   /* unpadded rows: one contiguous copy */
+			e = hipMemcpyAsync(j->Vb[s], src, (size_t)*r * row, hipMemcpyHostToDevice, j->cstream);
+		else
+			e = hipMemcpy2DAsync(j->Vb[s], j->ldv * 4, src, j->hldv * 4, row, *r, hipMemcpyHostToDevice,
+					     j->cstream);
 	} else if (!e) {
 		e = hipEventSynchronize(j->up[s]);
 		if (!e) {

@@ -266,3 +266,18 @@ def test_matrix_core_path_beyond_2GiB(nmfk, orc, k):
             assert r["iterations"] == 6
             out[path] = (j.get_W(), j.get_H())
     assert rel_err(out[2][0], out[1][0]) < 1e-4 and rel_err(out[2][1], out[1][1]) < 1e-4
+
+
+def test_transpose_beyond_65535_row_tiles(nmfk, orc):
+    # 4.3 million rows: more 64-row tiles than a grid's y extent holds, so the transposed
+    # copy of V (the W half's input) is built by tiles strided over the grid.
+    # One W half-step (it reads only the transposed copy) against the FP64 oracle.
+    n, m, k = 4_300_000, 8, 2
+    V = orc.gen_lognormal(n, m, 19, 0.0, 1.0)
+    with nmfk.Job(n, m, k) as j:
+        j.set_V(V)
+        j.init_random(5)
+        j.update_W()
+        W = j.get_W()
+    W0, H0 = orc.init_WH(n, m, k, 5)
+    assert rel_err(W, orc.update_W(V, W0, H0)) < 1e-5

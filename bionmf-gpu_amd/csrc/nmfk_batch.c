@@ -21,14 +21,17 @@
 #include <stdlib.h>
 #include <string.h>
 
-/* Partial tables per half and body class: VALU jobs (kt <= 8), matrix-core jobs of kt 16
- * and of kt 32, each launched on its own so that it runs at its own occupancy. */
-#define NCLS 3
+/* Partial tables per half and body class: VALU half-steps without and with lane groups
+ * (a factor with few rows), matrix-core jobs of kt 16 and of kt 32, each launched on its own
+ * so that it runs at its own occupancy. */
+#define NCLS 4
 enum { TAB_PART_H0, TAB_PART_W0 = TAB_PART_H0 + NCLS, TAB_FIN_H = TAB_PART_W0 + NCLS, TAB_FIN_W, TAB_TEST, NTAB };
 
-static int job_cls(const nmfk_job *j)
+static int half_cls(const nmfk_job *j, int half)
 {
-	return j->kt <= 8 ? 0 : j->kt == 16 ? 1 : 2;
+	if (j->kt <= 8)
+		return (half ? j->planW.lb : j->planH.lb) > 1 ? 1 : 0;
+	return j->kt == 16 ? 2 : 3;
 }
 
 struct nmfk_batch {
@@ -243,7 +246,7 @@ static void push_partial(nmfk_batch *b, nmfk_tile *t, uint32_t *n, int half, int
 {
 	uint32_t maxS = 0;
 	for (uint32_t s = 0; s < b->nslots; s++)
-		if (b->job[s] && job_cls(b->job[s]) == cls) {
+		if (b->job[s] && half_cls(b->job[s], half) == cls) {
 			const nmfk_hplan *p = half ? &b->job[s]->planW : &b->job[s]->planH;
 			push(t, n, s, p->nab, 0);   /* denominator tile first: the finish needs it */
 			if (p->S > maxS)
@@ -251,7 +254,7 @@ static void push_partial(nmfk_batch *b, nmfk_tile *t, uint32_t *n, int half, int
 		}
 	if (b->tune.order == 0) {
 		for (uint32_t s = 0; s < b->nslots; s++)
-			if (b->job[s] && job_cls(b->job[s]) == cls) {
+			if (b->job[s] && half_cls(b->job[s], half) == cls) {
 				const nmfk_hplan *p = half ? &b->job[s]->planW : &b->job[s]->planH;
 				for (uint32_t by = 0; by < p->S; by++)
 					for (uint32_t bx = 0; bx < p->nab; bx++)
@@ -260,7 +263,7 @@ static void push_partial(nmfk_batch *b, nmfk_tile *t, uint32_t *n, int half, int
 	} else {
 		for (uint32_t by = 0; by < maxS; by++)
 			for (uint32_t s = 0; s < b->nslots; s++)
-				if (b->job[s] && job_cls(b->job[s]) == cls) {
+				if (b->job[s] && half_cls(b->job[s], half) == cls) {
 					const nmfk_hplan *p = half ? &b->job[s]->planW : &b->job[s]->planH;
 					if (by < p->S)
 						for (uint32_t bx = 0; bx < p->nab; bx++)

@@ -53,6 +53,7 @@ typedef struct nmfk_hplan {
 	uint32_t ap;      /* a's per lane */
 	uint32_t nw;      /* waves per workgroup */
 	uint32_t path;    /* NMFK_PATH_* actually used */
+	uint32_t lb;      /* VALU: lane groups of a wave taking different rows of U (1, 2, 4, 8) */
 } nmfk_hplan;
 
 /* Launch-shape tunables of the planner (process-wide; jobs planned afterwards use them). */
@@ -155,8 +156,9 @@ typedef struct nmfk_tile {
 #define NMFK_BATCH_KTMAX 32
 
 /* half: 0 = H half (U = V, A = H, B = W), 1 = W half (U = Vt, A = W, B = H).  cls: the body
- * class of every tile's job, 0 = vector ALUs (kt 4/8), 1 = matrix cores at kt 16, 2 = matrix
- * cores at kt 32; tiles of other jobs are skipped. */
+ * class of every tile's half-step: 0 = vector ALUs (kt 4/8) without lane groups, 1 = vector
+ * ALUs with lane groups (plan lb > 1), 2 and 3 = matrix cores at kt 16 and 32; tiles of other
+ * classes are skipped. */
 int nmfk_launch_batch_partial(const nmfk_bjob *jobs, const nmfk_tile *tiles, uint32_t ntiles, int half,
 			      int cls, uint32_t grid, hipStream_t stream);
 int nmfk_launch_batch_finish(const nmfk_bjob *jobs, const nmfk_tile *tiles, uint32_t ntiles, int half,

@@ -212,6 +212,7 @@ struct half_args {
 	const float *colB;
 	int ncolB;
 	float *den;
+	int lb;   // VALU lane groups (see partial_valu_body)
 };
 
 // LDS floats the partial bodies need (VALU: NW/2 wave slots of AP*KT*64; MFMA: NB-1 wave
@@ -230,9 +231,15 @@ __host__ __device__ constexpr int smem_mfma(int kt, int na, int nb)
 // Column bx == nab is the denominator column: its first workgroup reduces the other
 // factor's column-sum partials into the denominators the finish kernel divides by, off
 // the critical path of the streaming workgroups; the rest return at once.
-template <int KT, int AP, int NW>
+// A wave's 64 lanes are LB groups of 64 / LB: lane (la, lb) owns rows a of A
+// (AP per lane) and takes every LB-th of the wave's rows of U, so that a factor with few
+// rows (p <= 64 AP / LB) still keeps every lane busy.  The groups' sums are added by a
+// fixed xor tree at the end, then the waves' by a fixed LDS tree.
+template <int KT, int AP, int NW, int LB = 1>
 __device__ __forceinline__ void partial_valu_body(int bx, int by, const half_args &h, float *__restrict__ smem)
 {
+	static_assert(LB == 1 || LB == 2 || LB == 4 || LB == 8, "lane groups: 1, 2, 4 or 8");
+	constexpr int LA = 64 / LB;
 	const float *__restrict__ U = h.U;
 	const size_t ldu = h.ldu;
 	const float *__restrict__ A = h.A;
@@ -245,7 +252,8 @@ __device__ __forceinline__ void partial_valu_body(int bx, int by, const half_arg
 	}
 	const int lane = threadIdx.x & 63;
 	const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-	const int a0 = (bx * 64 + lane) * AP;
+	const int la = lane % LA, lg = lane / LA;
+	const int a0 = (bx * LA + la) * AP;
 	const int avail = p - a0;
 
 	float av[AP][KT], acc[AP][KT];
@@ -278,13 +286,15 @@ __device__ __forceinline__ void partial_valu_body(int bx, int by, const half_arg
 				bv[c] = Bb[c];
 		}
 	};
+	// rows of this lane: b_begin + wave * LB + lg, then every NW * LB-th
+	constexpr int STEP = NW * LB;
 #pragma unroll
 	for (int d = 0; d < DR; d++)
-		fetch(b_begin + wave + d * NW, un[d], bn[d]);
-	for (int b = b_begin + wave; b < b_end; b += DR * NW) {
+		fetch(b_begin + wave * LB + lg + d * STEP, un[d], bn[d]);
+	for (int b = b_begin + wave * LB + lg; b < b_end; b += DR * STEP) {
 #pragma unroll
 		for (int d = 0; d < DR; d++) {
-			const int rb = b + d * NW;
+			const int rb = b + d * STEP;
 			if (rb >= b_end)
 				break;
 			float u[AP], bv[KT];
@@ -294,7 +304,7 @@ __device__ __forceinline__ void partial_valu_body(int bx, int by, const half_arg
 #pragma unroll
 			for (int c = 0; c < KT; c++)
 				bv[c] = bn[d][c];
-			fetch(rb + DR * NW, un[d], bn[d]);
+			fetch(rb + DR * STEP, un[d], bn[d]);
 #pragma unroll
 			for (int r = 0; r < AP; r++) {
 				float wh = 0.0f;
@@ -312,6 +322,17 @@ __device__ __forceinline__ void partial_valu_body(int bx, int by, const half_arg
 		}
 	}
 
+	// Fixed-order xor tree over the lane groups: every lane of a group ends with the sum
+	// (the two partners of a step add the same two values).
+	if constexpr (LB > 1) {
+#pragma unroll
+		for (int o = 32; o >= LA; o >>= 1)
+#pragma unroll
+			for (int r = 0; r < AP; r++)
+#pragma unroll
+				for (int c = 0; c < KT; c++)
+					acc[r][c] += __shfl_xor(acc[r][c], o, 64);
+	}
 	// Fixed-order tree over the NW waves of the workgroup.
 	constexpr int E = AP * KT;
 	float *__restrict__ red = smem;
@@ -336,7 +357,7 @@ __device__ __forceinline__ void partial_valu_body(int bx, int by, const half_arg
 		}
 		__syncthreads();
 	}
-	if (wave == 0) {
+	if (wave == 0 && lg == 0) {
 		float *dst = h.part + ((size_t)by * p + a0) * KT;
 #pragma unroll
 		for (int r = 0; r < AP; r++)
@@ -367,7 +388,7 @@ __device__ __forceinline__ bool xcd_tile(int gx, int total, int &bx, int &by)
 	return true;
 }
 
-template <int KT, int AP, int NW>
+template <int KT, int AP, int NW, int LB>
 __global__ __launch_bounds__(64 * NW) void k_partial(half_args h, const nmfk_state *__restrict__ st, int S)
 {
 	__shared__ float smem[smem_valu(KT, AP, NW)];
@@ -376,7 +397,7 @@ __global__ __launch_bounds__(64 * NW) void k_partial(half_args h, const nmfk_sta
 	int bx, by;
 	if (!xcd_tile(h.nab + 1, (h.nab + 1) * S, bx, by))
 		return;
-	partial_valu_body<KT, AP, NW>(bx, by, h, smem);
+	partial_valu_body<KT, AP, NW, LB>(bx, by, h, smem);
 }
 
 // --------------------------------------------------------------------------------------
@@ -1381,40 +1402,63 @@ __device__ __forceinline__ half_args batch_half(const nmfk_bjob &J, int half)
 	h.colB = hh ? J.colW : J.colH;
 	h.ncolB = hh ? J.ncolW : J.ncolH;
 	h.den = hh ? J.denW : J.denH;
+	h.lb = (int)pl.lb;
 	return h;
 }
 
-// Three batched partial kernels, one per body class, so each runs at its own occupancy:
-// VALU bodies (KT 4/8), the matrix-core body of KT 16 and that of KT 32; a batch launches
-// each over its own tile table.
+// VALU body of a batched half-step with lane groups (2, 4 or 8, from its plan).
+template <int KT>
+__device__ __forceinline__ void batch_valu_body(int bx, int by, const half_args &h, float *__restrict__ smem)
+{
+	switch (h.lb) {
+	case 8:
+		partial_valu_body<KT, 4, 4, 8>(bx, by, h, smem);
+		break;
+	case 4:
+		partial_valu_body<KT, 4, 4, 4>(bx, by, h, smem);
+		break;
+	default:
+		partial_valu_body<KT, 4, 4, 2>(bx, by, h, smem);
+		break;
+	}
+}
+
+// Four batched partial kernels, one per body class, so each runs at its own occupancy: the
+// VALU bodies (KT 4/8) of half-steps without lane groups, those with lane groups (few rows
+// of the updated factor: their variants need more registers), the matrix-core body of KT 16
+// and that of KT 32; a batch launches each over its own tile table.
 template <int CLS>
 constexpr int smem_batch_partial()
 {
-	return CLS == 0 ? smem_valu(8, 4, 4) : smem_mfma(CLS == 1 ? 16 : 32, 1, 4);
+	return CLS <= 1 ? smem_valu(8, 4, 4) : smem_mfma(CLS == 2 ? 16 : 32, 1, 4);
 }
 
 template <int CLS>
-__global__ __launch_bounds__(NMFK_WG, (CLS == 1 ? 4 : 1)) void k_batch_partial(const nmfk_bjob *__restrict__ jobs,
+__global__ __launch_bounds__(NMFK_WG, (CLS == 0 || CLS == 2 ? 4 : 1)) void k_batch_partial(const nmfk_bjob *__restrict__ jobs,
 							   const nmfk_tile *__restrict__ tiles, int ntiles, int half)
 {
 	__shared__ float smem[smem_batch_partial<CLS>()];
 	for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
 		const nmfk_tile tl = tiles[t];
 		const nmfk_bjob &J = jobs[tl.job];
-		// a job of another class would not fit this kernel's body: never run it here
-		const bool fits = CLS == 0 ? J.kt <= 8 : J.kt == (CLS == 1 ? 16u : 32u);
+		// a half-step of another class would not fit this kernel's body: never run it here
+		const uint32_t lb = half == 0 ? J.planH.lb : J.planW.lb;
+		const bool fits = CLS <= 1 ? (J.kt <= 8 && (lb > 1) == (CLS == 1)) : J.kt == (CLS == 2 ? 16u : 32u);
 		if (!fits || !J.live || J.st->converged)
 			continue;
 		const half_args h = batch_half(J, half);
 		if constexpr (CLS == 0) {
 			if (J.kt == 4)
-				partial_valu_body<4, 4, 4>(tl.bx, tl.by, h, smem);
+				partial_valu_body<4, 4, 4, 1>(tl.bx, tl.by, h, smem);
 			else
-				partial_valu_body<8, 4, 4>(tl.bx, tl.by, h, smem);
+				partial_valu_body<8, 4, 4, 1>(tl.bx, tl.by, h, smem);
 		} else if constexpr (CLS == 1) {
-			partial_mfma_body<16, 1, 4>(tl.bx, tl.by, h, smem);
+			if (J.kt == 4)
+				batch_valu_body<4>(tl.bx, tl.by, h, smem);
+			else
+				batch_valu_body<8>(tl.bx, tl.by, h, smem);
 		} else {
-			partial_mfma_body<32, 1, 4>(tl.bx, tl.by, h, smem);
+			partial_mfma_body<CLS == 2 ? 16 : 32, 1, 4>(tl.bx, tl.by, h, smem);
 		}
 		__syncthreads();   // the next tile reuses the LDS
 	}
@@ -1775,17 +1819,24 @@ extern "C" void nmfk_plan_half_target(uint32_t p, uint32_t q, uint32_t kt, int p
 		path = NMFK_PATH_MFMA32;
 	if (path != NMFK_PATH_VALU && kt % 16 != 0)
 		path = NMFK_PATH_VALU;
-	// rows a per workgroup = 64 * ap: VALU lanes own ap rows each; the MFMA kernel's NA
-	// side-by-side waves own 64 rows each (ap = NA), its NB wave groups split the rows of U.
+	// rows a per workgroup: VALU lanes own ap rows each, in 64 / lb lanes per group (the lb
+	// groups split the rows of U); the MFMA kernel's NA side-by-side waves own 64 rows each
+	// (ap = NA), its NB wave groups split the rows of U.
 	int na = g_tune[NMFK_TUNE_MFMA_NA], nb = g_tune[NMFK_TUNE_MFMA_NB];
 	if (!mfma_shape_ok(na, nb))
 		na = 1, nb = 4;
 	const bool valu = path == NMFK_PATH_VALU;
 	const uint32_t ap = valu ? default_ap(kt) : (uint32_t)na;
 	const uint32_t nw = valu ? 4u : (uint32_t)(na * nb);
-	const uint32_t rows_per_step = valu ? 1u : path == NMFK_PATH_MFMA_BF16 ? 32u : 16u;   // rows of U per wave step
+	// VALU lane groups: as many as leave every lane a row of A (p <= 64 ap / lb)
+	uint32_t lb = 1;
+	if (valu)
+		while (lb < 8 && p <= 64u * ap / (2u * lb))
+			lb *= 2;
+	const uint32_t rows_per_step = valu ? lb : path == NMFK_PATH_MFMA_BF16 ? 32u : 16u;   // rows of U per wave step
 	const uint32_t steppers = valu ? nw : (uint32_t)nb;        // wave groups splitting the rows
-	const uint32_t nab = (p + 64u * ap - 1u) / (64u * ap);
+	const uint32_t rows_per_wg = valu ? 64u / lb * ap : 64u * ap;   // rows a per workgroup
+	const uint32_t nab = (p + rows_per_wg - 1u) / rows_per_wg;
 	const uint32_t target =
 		target_wgs ? target_wgs : (uint32_t)g_tune[valu ? NMFK_TUNE_WGS_VALU : NMFK_TUNE_WGS_MFMA];
 	uint32_t S = (target + nab - 1u) / nab;
@@ -1806,6 +1857,7 @@ extern "C" void nmfk_plan_half_target(uint32_t p, uint32_t q, uint32_t kt, int p
 	pl->ap = ap;
 	pl->nw = nw;
 	pl->path = (uint32_t)path;
+	pl->lb = lb;
 }
 
 extern "C" void nmfk_plan_half(uint32_t p, uint32_t q, uint32_t kt, nmfk_hplan *pl)
@@ -1829,8 +1881,24 @@ static int launch_partial_t(const float *U, size_t ldu, const float *A, const fl
 	const dim3 grid = xcd_grid(pl);
 	if (pl->nw != 4)
 		return (int)hipErrorInvalidValue;
-	const half_args h = {U, ldu, A, B, (int)p, (int)q, (int)pl->bchunk, (int)pl->nab, part, colB, (int)ncolB, den};
-	hipLaunchKernelGGL((k_partial<KT, AP, 4>), grid, dim3(256), 0, stream, h, st, (int)pl->S);
+	const half_args h = {U, ldu, A, B, (int)p, (int)q, (int)pl->bchunk, (int)pl->nab, part, colB, (int)ncolB, den,
+			     (int)pl->lb};
+	switch (pl->lb) {
+	case 1:
+		hipLaunchKernelGGL((k_partial<KT, AP, 4, 1>), grid, dim3(256), 0, stream, h, st, (int)pl->S);
+		break;
+	case 2:
+		hipLaunchKernelGGL((k_partial<KT, AP, 4, 2>), grid, dim3(256), 0, stream, h, st, (int)pl->S);
+		break;
+	case 4:
+		hipLaunchKernelGGL((k_partial<KT, AP, 4, 4>), grid, dim3(256), 0, stream, h, st, (int)pl->S);
+		break;
+	case 8:
+		hipLaunchKernelGGL((k_partial<KT, AP, 4, 8>), grid, dim3(256), 0, stream, h, st, (int)pl->S);
+		break;
+	default:
+		return (int)hipErrorInvalidValue;
+	}
 	return (int)hipGetLastError();
 }
 
@@ -2035,17 +2103,23 @@ extern "C" int nmfk_launch_batch_partial(const nmfk_bjob *jobs, const nmfk_tile 
 {
 	if (ntiles == 0)
 		return 0;
-	if (cls == 0)
-		hipLaunchKernelGGL((k_batch_partial<0>), dim3(batch_grid(ntiles, grid)), dim3(NMFK_WG), 0, stream, jobs,
-				   tiles, (int)ntiles, half);
-	else if (cls == 1)
-		hipLaunchKernelGGL((k_batch_partial<1>), dim3(batch_grid(ntiles, grid)), dim3(NMFK_WG), 0, stream, jobs,
-				   tiles, (int)ntiles, half);
-	else if (cls == 2)
-		hipLaunchKernelGGL((k_batch_partial<2>), dim3(batch_grid(ntiles, grid)), dim3(NMFK_WG), 0, stream, jobs,
-				   tiles, (int)ntiles, half);
-	else
+	const dim3 g(batch_grid(ntiles, grid));
+	switch (cls) {
+	case 0:
+		hipLaunchKernelGGL((k_batch_partial<0>), g, dim3(NMFK_WG), 0, stream, jobs, tiles, (int)ntiles, half);
+		break;
+	case 1:
+		hipLaunchKernelGGL((k_batch_partial<1>), g, dim3(NMFK_WG), 0, stream, jobs, tiles, (int)ntiles, half);
+		break;
+	case 2:
+		hipLaunchKernelGGL((k_batch_partial<2>), g, dim3(NMFK_WG), 0, stream, jobs, tiles, (int)ntiles, half);
+		break;
+	case 3:
+		hipLaunchKernelGGL((k_batch_partial<3>), g, dim3(NMFK_WG), 0, stream, jobs, tiles, (int)ntiles, half);
+		break;
+	default:
 		return (int)hipErrorInvalidValue;
+	}
 	return (int)hipGetLastError();
 }
 

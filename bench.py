@@ -132,6 +132,8 @@ def bench_streamed(pkg, job, iters: int = 50) -> dict:
     from pageable memory (staged), against the resident job on the same start."""
     V = job.get_V()
     job.init_random(SEED_INIT)
+    job.run(iters, T, 1_000_000_000)          # warm-up (captures the iteration graph)
+    job.init_random(SEED_INIT)
     t0 = time.perf_counter()
     job.run(iters, T, 1_000_000_000)
     t_res = time.perf_counter() - t0

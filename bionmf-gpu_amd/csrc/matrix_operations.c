@@ -32,6 +32,8 @@ static void mark_if_V(real const *d_A)
 {
 	if (nmfk_dropin_job && d_A == nmfk_dropin_job->V)
 		nmfk_job_mark_V_changed(nmfk_dropin_job);
+	if (nmfk_dropin_job && nmfk_dropin_job->streamed && d_A == nmfk_dropin_job->Vb[0])
+		nmfk_dropin_job->slot0_written = 1;
 }
 
 index_t gpu_max_height(index_t pitch)

@@ -60,6 +60,7 @@ struct nmfk_job {
 	hipStream_t cstream;           /* host-to-device copies */
 	hipEvent_t up[2], done[2];     /* slot s: block uploaded / block's kernels finished */
 	double stream_bytes;           /* bytes of V copied to the device by the last run */
+	int slot0_written;             /* a caller wrote slot 0 on the job's stream (drop-in upload) */
 };
 
 /* Streamed jobs: the loop and the Frobenius/KL pass (nmfk_stream.c). */

@@ -292,11 +292,15 @@ static int pass(nmfk_job *j, int kind, const float *Hp, int clamp, double acc[3]
 			return NMFK_ENOMEM;
 		}
 	}
-	/* copies into the slots start after the job stream's earlier work (a caller's own
-	 * upload into slot 0 included) */
-	int e = hipEventRecord(j->ev[0], j->stream);
-	if (!e)
-		e = hipStreamWaitEvent(j->cstream, j->ev[0], 0);
+	/* after a caller's own upload into slot 0 (the drop-in driver's first block, on the
+	 * job's stream), copies into the slots wait for it */
+	int e = 0;
+	if (j->slot0_written) {
+		e = hipEventRecord(j->ev[0], j->stream);
+		if (!e)
+			e = hipStreamWaitEvent(j->cstream, j->ev[0], 0);
+		j->slot0_written = 0;
+	}
 	for (uint32_t b = 0; b < j->nblocks && !e; b++) {
 		const int s = (int)(b & 1u), lastb = b + 1 == j->nblocks;
 		uint32_t i0, r;

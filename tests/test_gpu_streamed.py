@@ -89,15 +89,16 @@ def test_streamed_golden_tiny_runs(nmfk, orc):
             assert np.array_equal(job.get_classification(), g[p + "cls"]), ci
 
 
-def test_streamed_pinned_and_staged_give_same_bits(nmfk, orc):
-    n, m, k = 2000, 170, 12
+@pytest.mark.parametrize("n,m,k,br,iters", [(2000, 170, 12, 384, 120), (20000, 1000, 8, 4096, 20)])
+def test_streamed_pinned_and_staged_give_same_bits(nmfk, orc, n, m, k, br, iters):
+    # The second case stages 16 MB blocks, copied by several host threads.
     V = orc.gen_lognormal(n, m, 77, 0.0, 1.0)
-    a = nmfk.Job.streamed(V, k, 384, pinned=True)
-    b = nmfk.Job.streamed(V, k, 384, pinned=False)
+    a = nmfk.Job.streamed(V, k, br, pinned=True)
+    b = nmfk.Job.streamed(V, k, br, pinned=False)
     for j in (a, b):
         j.init_random(5)
-    ra, rb = a.run(120, 10, 1000), b.run(120, 10, 1000)
-    assert ra["iterations"] == rb["iterations"] == 120
+    ra, rb = a.run(iters, 10, 1000), b.run(iters, 10, 1000)
+    assert ra["iterations"] == rb["iterations"] == iters
     assert np.array_equal(a.get_W(), b.get_W()) and np.array_equal(a.get_H(), b.get_H())
     a.close(), b.close()
 

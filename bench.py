@@ -64,6 +64,72 @@ def cpu_baseline(sample_iters: int) -> dict:
                       f"(s/iter {s_iter:.4f}) extrapolated to {ITERS} iterations"}
 
 
+def cpu_configs(sweep: dict | None) -> dict:
+    """The oracle's FP32 scalar rendering of the reference update sequence on the host cores
+    for the other configurations (TEST INFRASTRUCTURE, timed as reported baselines only):
+    the 5000 x 38 k = 3 factorization in full on 1 and on all threads (the seeds of
+    bench_c0, so the same factorization); the 20000 x 1000 k = 16
+    loop on 1 thread (3 iterations); the rank sweep serially, from each rank's s/iteration
+    (20 iterations) times the sweep's own iteration counts; the 50000 x 2000 k = 64 loop
+    (2 iterations)."""
+    exe = ROOT / "oracle" / "nmf_cpu_baseline"
+    if not exe.exists():
+        return None
+    threads = min(16, os.cpu_count() or 1)
+
+    def run(n, m, k, iters, th, gen, S=1_000_000_000, **kw):
+        cmd = [str(exe), "--n", str(n), "--m", str(m), "--k", str(k), "--iters", str(iters), "--T", "10",
+               "--S", str(S), "--gen", gen, "--threads", str(th)]
+        for a, v in kw.items():
+            cmd += ["--" + a.replace("_", "-"), str(v)]
+        out = subprocess.run(cmd, check=True, capture_output=True, text=True, timeout=600).stdout
+        return json.loads(out.strip().splitlines()[-1])
+
+    out = {"kind": "port", "threads_all": threads}
+    c0 = {}
+    for th in (1, threads):
+        r = run(5000, 38, 3, 2000, th, "lognormal", S=40, mu=5.0, sigma=1.5, seed_v=5, seed_init=7)
+        c0[f"threads_{th}"] = {"factorizations_per_s": 1.0 / r["seconds"], "iterations": r["iters_done"],
+                               "converged": bool(r["converged"])}
+    out["c0_5000x38_k3_full_run"] = c0
+    r = run(N, M, K, 3, 1, "planted", kstar=KSTAR, sigma=SIGMA)
+    out["c1_20000x1000_k16_one_thread"] = {"s_per_iteration": r["s_per_iter"], "sample": "3 iterations"}
+    if sweep and "mean_iterations_by_k" in sweep:
+        total = 0.0
+        per_k = {}
+        for k_str, iters in sweep["mean_iterations_by_k"].items():
+            r = run(5000, 200, int(k_str), 20, threads, "planted", kstar=6, sigma=0.3)
+            per_k[k_str] = r["s_per_iter"]
+            total += 20 * iters * r["s_per_iter"]
+        out["c2_rank_sweep_serial"] = {"jobs_per_s": 180.0 / total, "s_per_iteration_by_k": per_k,
+                                       "sample": "20 iterations per rank on all threads, times the sweep's "
+                                                 "mean iterations per rank and 20 restarts"}
+    r = run(50000, 2000, 64, 2, threads, "planted", kstar=48, sigma=0.3)
+    out["c3_50000x2000_k64"] = {"s_per_iteration": r["s_per_iter"], "sample": "2 iterations on all threads"}
+    return out
+
+
+def bench_c0(pkg, runs: int = 5) -> dict:
+    """The reference's CPU-runnable case: V 5000 x 38 lognormal(5, 1.5^2), k = 3, stop rule
+    T = 10, 40 unchanged tests, 2000 iterations at most; whole factorizations (median of
+    <runs>), V resident."""
+    job = pkg.Job(5000, 38, 3)
+    job.gen_V_lognormal(5, 5.0, 1.5)
+    job.init_random(7)
+    job.run(2000, 10, 40)            # warm-up (captures the iteration graph)
+    times, iters = [], 0
+    for r in range(runs):
+        job.init_random(7)
+        t0 = time.perf_counter()
+        res = job.run(2000, 10, 40)
+        times.append(time.perf_counter() - t0)
+        iters = res["iterations"]
+    job.close()
+    t = float(np.median(times))
+    return {"workload": "V 5000x38 lognormal(5, 1.5^2), k=3, T=10, stop_threshold=40, nIters=2000",
+            "factorizations_per_s": 1.0 / t, "iterations": iters, "ms_per_iteration": 1e3 * t / max(iters, 1)}
+
+
 def bench_sweep(pkg) -> dict:
     """Rank sweep: V 5000 x 200 planted rank 6, k = 2..10 x 20 restarts = 180 jobs, stop rule
     (T=10, 40, 2000 iterations); batched (one launch per phase for all jobs, launch shape
@@ -87,6 +153,8 @@ def bench_sweep(pkg) -> dict:
             "one_at_a_time_default_plan_jobs_per_s": nj / c["seconds"],
             "identical_class_vectors": bool(np.array_equal(a["cls"], b["cls"])) and same,
             "mean_iterations": float(np.mean([r["iterations"] for r in a["results"]])),
+            "mean_iterations_by_k": {str(k): float(np.mean([r["iterations"] for r in a["results"][q * R:(q + 1) * R]]))
+                                     for q, k in enumerate(ks)},
             "converged_jobs": int(sum(r["converged"] for r in a["results"])),
             "autotune_seconds": a["tune_seconds"], "tuning": a["tuning"], "plan_target": a["plan_target"]}
 
@@ -259,6 +327,7 @@ def main():
 
     extra = {}
     if not args.no_extra:
+        extra["c0"] = bench_c0(pkg)
         extra["sweep"] = bench_sweep(pkg)
         extra["dense_rank"] = bench_dense_rank(pkg)
         extra["served"] = bench_served(pkg, args.served_requests, args.served_rate)
@@ -270,6 +339,10 @@ def main():
             cpu = cpu_baseline(args.cpu_iters)
         except Exception as e:  # reported, never fatal for the GPU number
             cpu = {"error": str(e)}
+        try:
+            extra["cpu_configs"] = cpu_configs(extra.get("sweep"))
+        except Exception as e:
+            extra["cpu_configs"] = {"error": str(e)}
 
     line = {
         "metric": "NMF factorizations/sec at <=1e-3 rel KL err; p50/p99 job latency; % HBM roofline",

@@ -233,7 +233,7 @@ def gpu_count() -> int:
     return n.value if rc == 0 else 0
 
 
-TUNE_WGS_VALU, TUNE_WGS_MFMA, TUNE_MIN_STEPS, TUNE_MAX_SPLITS, TUNE_MFMA_NA, TUNE_MFMA_NB = 0, 1, 2, 3, 4, 5
+TUNE_WGS_VALU, TUNE_WGS_MFMA, TUNE_MIN_STEPS, TUNE_MAX_SPLITS, TUNE_MFMA_NA, TUNE_MFMA_NB, TUNE_WGS_BF16 = range(7)
 
 
 def tune_set(key: int, value: int):

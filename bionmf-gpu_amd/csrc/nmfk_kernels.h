@@ -63,7 +63,8 @@ typedef struct nmfk_hplan {
 #define NMFK_TUNE_MAX_SPLITS 3   /* most splits of the contraction range (partials to reduce) */
 #define NMFK_TUNE_MFMA_NA 4      /* MFMA workgroup: waves side by side along the updated factor */
 #define NMFK_TUNE_MFMA_NB 5      /* MFMA workgroup: wave groups splitting the rows of U */
-#define NMFK_TUNE_COUNT 6
+#define NMFK_TUNE_WGS_BF16 6     /* workgroups the BF16 partial kernel aims for */
+#define NMFK_TUNE_COUNT 7
 int nmfk_tune_set(int key, int value);
 int nmfk_tune_get(int key);
 

@@ -1784,7 +1784,10 @@ static inline uint32_t default_ap(uint32_t kt)
 
 // Launch-shape tunables (workgroups the partial kernels aim for, minimum steps per wave,
 // most splits, and the MFMA workgroup shape NA x NB).
-static int g_tune[NMFK_TUNE_COUNT] = {512, 1024, 8, 64, 1, 4};
+// The BF16 kernel (kt 64) runs 2 waves per SIMD: 512 four-wave workgroups fill the 256
+// CUs once.  At 50000 x 2000, k = 64 its H half takes 105 us with the 16 splits that gives
+// against 115 us with the 32 of the FP32 target.
+static int g_tune[NMFK_TUNE_COUNT] = {512, 1024, 8, 64, 1, 4, 512};
 
 static bool mfma_shape_ok(int na, int nb)
 {
@@ -1838,7 +1841,9 @@ extern "C" void nmfk_plan_half_target(uint32_t p, uint32_t q, uint32_t kt, int p
 	const uint32_t rows_per_wg = valu ? 64u / lb * ap : 64u * ap;   // rows a per workgroup
 	const uint32_t nab = (p + rows_per_wg - 1u) / rows_per_wg;
 	const uint32_t target =
-		target_wgs ? target_wgs : (uint32_t)g_tune[valu ? NMFK_TUNE_WGS_VALU : NMFK_TUNE_WGS_MFMA];
+		target_wgs ? target_wgs
+			   : (uint32_t)g_tune[valu ? NMFK_TUNE_WGS_VALU
+						   : path == NMFK_PATH_MFMA_BF16 ? NMFK_TUNE_WGS_BF16 : NMFK_TUNE_WGS_MFMA];
 	uint32_t S = (target + nab - 1u) / nab;
 	const uint32_t min_rows = steppers * rows_per_step * (uint32_t)g_tune[NMFK_TUNE_MIN_STEPS];
 	const uint32_t smax = (q + min_rows - 1u) / min_rows;

@@ -99,17 +99,19 @@ int nmfk_job_dot_kl(nmfk_job *job, double *dot_V, double *dot_VWH, double *kl);
 int nmfk_job_sync(nmfk_job *job);
 
 /* Kernel path of the half-step contractions: 0 = automatic (vector ALUs for k <= 8, FP32
- * matrix cores above), 1 = vector ALUs, 2 = FP32 matrix cores (k padded to 16). */
+ * matrix cores for k 9..32, BF16 matrix cores above), 1 = vector ALUs, 2 = FP32 matrix
+ * cores (k padded to 16), 3 = BF16 matrix cores (k padded to 32; FP32 accumulation). */
 int nmfk_job_set_kernel_path(nmfk_job *job, int path);
 int nmfk_job_kernel_path(const nmfk_job *job);
 
 /* Launch-shape tunables of the kernel planner, process-wide; they apply to jobs created
  * afterwards.  Keys: 0 = workgroups the vector-ALU partial kernel aims for (default 512),
- * 1 = workgroups the matrix-core partial kernel aims for (default 1024), 2 = minimum row
- * steps per wave before the contraction range is split further (default 8), 3 = most
+ * 1 = workgroups the FP32 matrix-core partial kernel aims for (default 1024), 2 = minimum
+ * row steps per wave before the contraction range is split further (default 8), 3 = most
  * splits of the contraction range (default 64), 4 and 5 = matrix-core workgroup shape:
  * waves side by side along the updated factor x wave groups splitting the contraction
- * range (default 1 x 4; also 2x2, 4x1, 2x4, 4x2, 1x8). */
+ * range (default 1 x 4; also 2x2, 4x1, 2x4, 4x2), 6 = workgroups the BF16 matrix-core
+ * partial kernel aims for (default 512). */
 int nmfk_tune_set(int key, int value);
 int nmfk_tune_get(int key);
 

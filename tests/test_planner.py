@@ -1,0 +1,108 @@
+"""The launch planner of the half-step kernels (host code, no GPU): which kernel path a rank
+gets, how the contraction range is split, and the process-wide tunables it reads."""
+from __future__ import annotations
+
+import ctypes as C
+
+import pytest
+
+PATH_AUTO, PATH_VALU, PATH_MFMA32, PATH_BF16 = 0, 1, 2, 3
+DEFAULTS = {0: 512, 1: 1024, 2: 8, 3: 64, 4: 1, 5: 4, 6: 512}
+
+
+class HPlan(C.Structure):
+    _fields_ = [(f, C.c_uint32) for f in ("nab", "S", "bchunk", "ap", "nw", "path", "lb")]
+
+
+@pytest.fixture
+def lib(pkg):
+    lib = pkg.load()
+    lib.nmfk_plan_half_path.restype = None
+    lib.nmfk_plan_half_path.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, C.POINTER(HPlan)]
+    saved = {key: lib.nmfk_tune_get(key) for key in DEFAULTS}
+    yield lib
+    for key, value in saved.items():
+        lib.nmfk_tune_set(key, value)
+
+
+def plan(lib, p, q, kt, path=PATH_AUTO):
+    pl = HPlan()
+    lib.nmfk_plan_half_path(p, q, kt, path, C.byref(pl))
+    return pl
+
+
+def test_tunable_defaults_and_bounds(lib):
+    assert {key: lib.nmfk_tune_get(key) for key in DEFAULTS} == DEFAULTS
+    assert lib.nmfk_tune_get(7) == -1 and lib.nmfk_tune_get(-1) == -1
+    assert lib.nmfk_tune_set(7, 1) != 0 and lib.nmfk_tune_set(1, 0) != 0
+    assert lib.nmfk_tune_get(1) == 1024
+
+
+@pytest.mark.parametrize("kt,path", [(4, PATH_VALU), (8, PATH_VALU), (16, PATH_MFMA32), (32, PATH_MFMA32),
+                                     (64, PATH_BF16)])
+def test_automatic_path_by_rank(lib, kt, path):
+    assert plan(lib, 1000, 20000, kt).path == path
+
+
+def test_requested_paths_fall_back_where_the_kernel_cannot_run(lib):
+    assert plan(lib, 1000, 20000, 16, PATH_BF16).path == PATH_MFMA32   # BF16 needs kt % 32 == 0
+    assert plan(lib, 1000, 20000, 8, PATH_MFMA32).path == PATH_VALU    # matrix cores need kt % 16 == 0
+    assert plan(lib, 1000, 20000, 64, PATH_MFMA32).path == PATH_MFMA32
+    assert plan(lib, 1000, 20000, 32, PATH_VALU).path == PATH_VALU
+
+
+SHAPES = [(1, 1), (3, 7), (38, 5000), (5000, 38), (200, 5000), (1000, 20000), (20000, 1000), (2000, 50000),
+          (50000, 2000), (60000, 20), (20, 60000), (3000, 4_300_000)]
+
+
+@pytest.mark.parametrize("p,q", SHAPES)
+@pytest.mark.parametrize("kt,path", [(4, PATH_VALU), (8, PATH_VALU), (16, PATH_VALU), (16, PATH_MFMA32),
+                                     (32, PATH_MFMA32), (64, PATH_MFMA32), (64, PATH_BF16)])
+def test_plan_covers_both_ranges(lib, p, q, kt, path):
+    pl = plan(lib, p, q, kt, path)
+    assert pl.path == path
+    if path == PATH_VALU:
+        step, rows_a = pl.lb, 64 // pl.lb * pl.ap
+        assert pl.nw == 4 and pl.lb in (1, 2, 4, 8)
+        # lane groups: the largest count that still gives every lane a row of the factor
+        # (none for a factor of more than 32 ap rows)
+        assert pl.lb == 1 or p <= 64 * pl.ap // pl.lb
+        assert pl.lb == 8 or p > 64 * pl.ap // (2 * pl.lb)
+    else:
+        step, rows_a = (32 if path == PATH_BF16 else 16), 64 * pl.ap
+        assert (pl.ap, pl.nw) == (1, 4) and pl.lb == 1
+    assert pl.nab == -(-p // rows_a)
+    assert pl.bchunk % step == 0 and pl.S >= 1 and pl.S <= DEFAULTS[3]
+    assert (pl.S - 1) * pl.bchunk < q <= pl.S * pl.bchunk   # every split holds rows, together all q
+
+
+def test_bf16_plans_of_the_dense_rank_workload(lib):
+    # V 50000 x 2000, k = 64: the H half (rows of H against the 50000 rows of V) in 16 splits
+    # of 3136 rows, 33 x 16 = 528 workgroups (with the denominator column); the W half in one
+    # split over the 2000 columns.
+    h, w = plan(lib, 2000, 50000, 64), plan(lib, 50000, 2000, 64)
+    assert (h.path, h.nab, h.S, h.bchunk) == (PATH_BF16, 32, 16, 3136)
+    assert (w.path, w.nab, w.S, w.bchunk) == (PATH_BF16, 782, 1, 2016)
+    # the FP32 matrix-core kernel keeps its own target of 1024 workgroups
+    assert plan(lib, 2000, 50000, 64, PATH_MFMA32).S == 32
+
+
+def test_tunables_move_the_plan(lib):
+    # c1's H half, 16 workgroup columns.  The minimum of 8 steps per wave caps the splits at
+    # ceil(20000 / (4 * 16 * 8)) = 40 under the 1024-workgroup target.  512 workgroups ask
+    # for 32 splits: 625 rows, rounded up to 640 (16-row steps), 32 splits.  2048 workgroups
+    # with 1 step per wave ask for 128, capped at 64: 313 rows -> 320, 63 splits.
+    base = plan(lib, 1000, 20000, 16)
+    assert (base.nab, base.S, base.bchunk) == (16, 40, 512)
+    assert lib.nmfk_tune_set(1, DEFAULTS[1] // 2) == 0
+    assert (plan(lib, 1000, 20000, 16).S, plan(lib, 1000, 20000, 16).bchunk) == (32, 640)
+    assert lib.nmfk_tune_set(2, 1) == 0 and lib.nmfk_tune_set(1, 2 * DEFAULTS[1]) == 0
+    assert (plan(lib, 1000, 20000, 16).S, plan(lib, 1000, 20000, 16).bchunk) == (63, 320)
+    assert lib.nmfk_tune_set(6, 256) == 0
+    assert plan(lib, 2000, 50000, 64).S == 8
+    assert lib.nmfk_tune_set(4, 2) == 0 and lib.nmfk_tune_set(5, 2) == 0
+    pl = plan(lib, 1000, 20000, 16)
+    assert (pl.ap, pl.nw, pl.nab) == (2, 4, 8)
+    assert lib.nmfk_tune_set(4, 3) == 0                     # 3 x 2 is no kernel shape: default 1 x 4
+    pl = plan(lib, 1000, 20000, 16)
+    assert (pl.ap, pl.nw) == (1, 4)

@@ -35,6 +35,7 @@ HBM_PEAK_GBS = 8000.0
 
 N, M, K, KSTAR, SIGMA, ITERS, T = 20000, 1000, 16, 16, 0.3, 500, 10
 SEED_V, SEED_INIT = 20250101, 7
+KERNEL_PATHS = {"auto": 0, "valu": 1, "mfma32": 2}   # nmfk_job_set_kernel_path
 
 
 def load_package():
@@ -256,6 +257,9 @@ def main():
                     help="run only the out-of-core (V in host memory) workload and print its line")
     ap.add_argument("--dense-only", action="store_true",
                     help="run only the dense-rank (k = 64, BF16 vs FP32) workload and print its line")
+    ap.add_argument("--kernel-path", choices=sorted(KERNEL_PATHS), default="auto",
+                    help="contraction kernels of the single large job (auto: FP32 matrix cores at k = 16); "
+                         "valu times the vector-ALU kernels on the same workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only (the multi-GPU row-block split "
@@ -282,6 +286,7 @@ def main():
 
     job = pkg.Job(N, M, K)
     job.gen_V_planted(SEED_V, KSTAR, SIGMA)
+    job.set_kernel_path(KERNEL_PATHS[args.kernel_path])
     if args.streamed_only:
         o = bench_streamed(pkg, job)
         print(json.dumps({"metric": "out-of-core: ms per iteration with V streamed from host memory",
@@ -322,7 +327,7 @@ def main():
     read_gbs, copy_gbs = pkg.measure_stream(1 << 30, 10, 4096)
     traffic = None
     tfile = ROOT / "profiles" / "traffic_partial_mfma_20000x1000_k16.json"
-    if tfile.exists():
+    if tfile.exists() and args.kernel_path == "auto":
         traffic = json.loads(tfile.read_text()).get("bytes_per_launch")
 
     extra = {}
@@ -368,7 +373,8 @@ def main():
         "frobenius_ratio": float(np.sqrt(dvwh / dv)),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_partial_mfma<16,1,4> (both half-steps)",
+                     "kernel": ("k_partial_mfma<16,1,4>" if args.kernel_path in ("auto", "mfma32")
+                                else "k_partial<16,...> (vector ALUs)") + " (both half-steps)",
                      "bytes_per_launch": bytes_partial, "ms_per_launch": ms,
                      "measured_read_GBps": read_gbs, "measured_copy_GBps": copy_gbs,
                      "frac_of_measured_read": achieved / read_gbs},

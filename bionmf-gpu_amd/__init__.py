@@ -17,6 +17,7 @@ from __future__ import annotations
 import ctypes as C
 import os
 import pathlib
+import weakref
 
 import numpy as np
 
@@ -291,12 +292,16 @@ class Job:
         self.lib = load()
         self.n, self.m, self.k = int(n), int(m), int(k)
         self._src = _src
+        self._sharers = weakref.WeakSet()   # open jobs reading this job's V
+        self._batch = None                  # the Batch holding this job, until poll returns it
         if _src is None:
             self.h = self.lib.nmfk_job_create(self.n, self.m, self.k)
         else:
             self.h = self.lib.nmfk_job_create_shared(_src.h, self.k)
         if not self.h:
             raise NmfkError(f"nmfk_job_create failed: {last_error()}")
+        if _src is not None:
+            _src._sharers.add(self)
 
     @classmethod
     def shared(cls, src: "Job", k: int) -> "Job":
@@ -312,6 +317,7 @@ class Job:
         V = np.ascontiguousarray(V, dtype=np.float32)
         self.n, self.m = (int(x) for x in V.shape)
         self.k, self._src, self._pin = int(k), None, None
+        self._sharers, self._batch = weakref.WeakSet(), None
         if pinned:
             self._pin = PinnedArray(V.shape)
             self._pin.array[...] = V
@@ -340,7 +346,17 @@ class Job:
         return {"H": dict(zip(keys, map(int, out[:4]))), "W": dict(zip(keys, map(int, out[4:])))}
 
     def close(self):
+        """Frees the job.  Refused while a batch still holds it (until Batch.poll returns it
+        or the batch is closed) or while jobs made by Job.shared(self, k) are open: both
+        would go on reading its device memory."""
         if self.h:
+            batch = getattr(self, "_batch", None)
+            if batch is not None and batch.h and self.h in batch._jobs:
+                raise NmfkError("job is in an active batch: poll it out or close the batch first")
+            open_sharers = [j for j in getattr(self, "_sharers", ()) if j.h]
+            if open_sharers:
+                raise NmfkError(f"{len(open_sharers)} job(s) made by Job.shared still read this job's V: "
+                                "close them first")
             self.lib.nmfk_job_destroy(self.h)
             self.h = None
         pin = getattr(self, "_pin", None)
@@ -462,6 +478,9 @@ class Batch:
         if self.h:
             self.lib.nmfk_batch_destroy(self.h)
             self.h = None
+            for job in self._jobs.values():
+                job._batch = None
+            self._jobs.clear()
 
     def __del__(self):
         try:
@@ -475,6 +494,7 @@ class Batch:
         if slot < 0:
             raise NmfkError(f"batch add failed: {last_error()}")
         self._jobs[job.h] = job
+        job._batch = self
         return slot
 
     @property
@@ -490,7 +510,10 @@ class Batch:
         res = (Result * self.cap)()
         nd = C.c_uint32(0)
         _check(self.lib.nmfk_batch_poll(self.h, done, res, self.cap, C.byref(nd)), "batch poll")
-        return [(self._jobs.pop(done[i]), res[i].as_dict()) for i in range(nd.value)]
+        out = [(self._jobs.pop(done[i]), res[i].as_dict()) for i in range(nd.value)]
+        for job, _ in out:
+            job._batch = None
+        return out
 
     def tuning(self) -> dict:
         t = BatchTuning()

@@ -155,7 +155,9 @@ int nmfk_job_plan_get(const nmfk_job *job, uint32_t out[8]);
  * the iteration; each job's results are bitwise those of nmfk_job_run on it.  Jobs join
  * with nmfk_batch_add (W and H already initialised) and leave through nmfk_batch_poll
  * once the stop rule is met or nIters is reached; after poll returns a job, the batch no
- * longer touches it.  All jobs of a batch share niter_test_conv.  Not thread-safe.
+ * longer touches it.  A job must not be destroyed while it is in a batch (added and not yet
+ * returned by poll), unless the batch itself has been destroyed first.  All jobs of a batch
+ * share niter_test_conv.  Not thread-safe.
  * ------------------------------------------------------------------------------------- */
 typedef struct nmfk_batch nmfk_batch;
 

@@ -203,3 +203,34 @@ def test_full_sweep_matches_fp64(nmfk, orc):
         kl, klo = orc.kl_div(V, W, H), orc.kl_div(V, o["W"], o["H"])
         assert abs(kl - klo) / klo < 1e-3, k
     src.close()
+
+
+def test_jobs_are_not_freed_under_a_batch_or_a_sharer(nmfk, orc):
+    # Closing a job that a batch still steps, or whose V a shared job still reads, would
+    # free device memory in use: the bindings refuse it until the job is polled out (or the
+    # batch closed) and the sharers are closed.
+    V = orc.gen_lognormal(300, 40, 3, 0.0, 1.0)
+    src = nmfk.Job(300, 40, 3)
+    src.set_V(V)
+    src.init_random(1)
+    twin = nmfk.Job.shared(src, 4)
+    twin.init_random(2)
+    with pytest.raises(nmfk.NmfkError, match="Job.shared"):
+        src.close()
+    batch = nmfk.Batch(T, 4)
+    batch.add(twin, 50, 4)
+    with pytest.raises(nmfk.NmfkError, match="active batch"):
+        twin.close()
+    out = batch.run()
+    assert [j for j, _ in out] == [twin]
+    twin.close()
+    src.close()
+    assert src.h is None and twin.h is None
+    j = nmfk.Job(300, 40, 3)
+    j.set_V(V)
+    j.init_random(3)
+    other = nmfk.Batch(T, 2)
+    other.add(j, 50, 4)
+    other.close()                    # the batch goes first: the job may then be freed
+    j.close()
+    batch.close()

@@ -195,6 +195,32 @@ def bench_dense_rank(pkg) -> dict:
                              "class_agreement": float(np.mean(b["cls"] == f["cls"]))}}
 
 
+def bench_row_block_shards(pkg, gpus: int = 8) -> dict:
+    """One GPU's share of a row-block split of V over <gpus> GPUs, at its real shape: the
+    dense rank (50000 x 2000, k = 64, BF16 contractions) and the headline job (20000 x 1000,
+    k = 16).  Each GPU would run these kernels on its rows and all-reduce the H half's
+    partial sums (m x k floats) plus colsum(W) (k floats) once per iteration; the all-reduce
+    is not run here (one GPU)."""
+    out = {}
+    for name, (n, m, k, path, kstar) in {"dense_rank": (50000, 2000, 64, 3, 48),
+                                         "single_large": (20000, 1000, 16, 0, 16)}.items():
+        rows = n // gpus
+        job = pkg.Job(rows, m, k)
+        job.gen_V_planted(SEED_V + rows, kstar, SIGMA)
+        if path:
+            job.set_kernel_path(path)
+        job.init_random(SEED_INIT)
+        tm = min((job.time_iterations(20) for _ in range(3)), key=lambda t: t["ms_per_iter"])
+        job.close()
+        part_bytes = 4.0 * (rows * m + rows * k + m * k)
+        out[name] = {"shard": f"V {rows}x{m} (1/{gpus} of {n} rows), k={k}",
+                     "ms_per_iteration": tm["ms_per_iter"],
+                     "us_partial_H": 1e3 * tm["ms_partial_H"], "us_partial_W": 1e3 * tm["ms_partial_W"],
+                     "partial_GBps": part_bytes / (0.5e-3 * (tm["ms_partial_H"] + tm["ms_partial_W"])) / 1e9,
+                     "allreduce_bytes_per_iteration": 4 * (m * k + k)}
+    return out
+
+
 def bench_streamed(pkg, job, iters: int = 50) -> dict:
     """Out-of-core path: the headline V (20000 x 1000, k = 16) kept in host memory and
     streamed to the device in 4096-row blocks once per iteration, from pinned memory and
@@ -255,6 +281,8 @@ def main():
                     help="run only the served stream (e.g. the full 2000-request stream) and print its line")
     ap.add_argument("--streamed-only", action="store_true",
                     help="run only the out-of-core (V in host memory) workload and print its line")
+    ap.add_argument("--shards-only", action="store_true",
+                    help="run only one GPU's row block of an 8-way split (dense rank and headline shapes)")
     ap.add_argument("--dense-only", action="store_true",
                     help="run only the dense-rank (k = 64, BF16 vs FP32) workload and print its line")
     ap.add_argument("--kernel-path", choices=sorted(KERNEL_PATHS), default="auto",
@@ -277,6 +305,12 @@ def main():
         print(json.dumps({"metric": "dense rank: factorizations/s (BF16 contractions)",
                           "value": d["bf16_factorizations_per_s"], "unit": "factorizations/s", "n_gpus": 1,
                           "higher_is_better": True, "dense_rank": d}))
+        return
+    if args.shards_only:
+        r = bench_row_block_shards(pkg)
+        print(json.dumps({"metric": "one GPU's row block of an 8-way split: ms per iteration",
+                          "value": r["dense_rank"]["ms_per_iteration"], "unit": "ms/iteration", "n_gpus": 1,
+                          "higher_is_better": False, "row_block_shards": r}))
         return
     if args.served_only:
         s = bench_served(pkg, args.served_requests, args.served_rate)
@@ -337,6 +371,7 @@ def main():
         extra["dense_rank"] = bench_dense_rank(pkg)
         extra["served"] = bench_served(pkg, args.served_requests, args.served_rate)
         extra["streamed"] = bench_streamed(pkg, job)
+        extra["row_block_shards"] = bench_row_block_shards(pkg)
 
     cpu = None
     if not args.no_cpu_baseline:

@@ -289,6 +289,8 @@ def main():
                     help="contraction kernels of the single large job (auto: FP32 matrix cores at k = 16); "
                          "valu times the vector-ALU kernels on the same workload")
     args = ap.parse_args()
+    if args.steps < 1 or args.warmup < 0:
+        ap.error("--steps must be >= 1 and --warmup >= 0")
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only (the multi-GPU row-block split "
                                    "of V is designed, not built); --gpus must be 1"}))

@@ -221,6 +221,28 @@ def bench_row_block_shards(pkg, gpus: int = 8) -> dict:
     return out
 
 
+def bench_size_sweep(pkg) -> list:
+    """The headline's kernels (k = 16, FP32 matrix cores) on planted inputs of growing height
+    at m = 1000 and 2000: the partial launches' duration against their algorithmic bytes.
+    The slope between sizes is the kernel's marginal streaming rate; the intercept is the
+    fixed cost of one launch."""
+    out = []
+    for m in (1000, 2000):
+        for n in (10000, 20000, 40000, 80000, 160000):
+            job = pkg.Job(n, m, K)
+            job.gen_V_planted(SEED_V + n + m, KSTAR, SIGMA)
+            job.init_random(SEED_INIT)
+            tm = min((job.time_iterations(20) for _ in range(3)), key=lambda t: t["ms_per_iter"])
+            plan = job.plan()
+            job.close()
+            b = 4.0 * (n * m + n * K + m * K)
+            out.append({"n": n, "m": m, "bytes_per_launch": b,
+                        "us_partial_H": 1e3 * tm["ms_partial_H"], "us_partial_W": 1e3 * tm["ms_partial_W"],
+                        "GBps_H": b / (tm["ms_partial_H"] * 1e-3) / 1e9, "GBps_W": b / (tm["ms_partial_W"] * 1e-3) / 1e9,
+                        "splits_H": plan["H"]["S"], "splits_W": plan["W"]["S"]})
+    return out
+
+
 def bench_streamed(pkg, job, iters: int = 50) -> dict:
     """Out-of-core path: the headline V (20000 x 1000, k = 16) kept in host memory and
     streamed to the device in 4096-row blocks once per iteration, from pinned memory and
@@ -281,6 +303,8 @@ def main():
                     help="run only the served stream (e.g. the full 2000-request stream) and print its line")
     ap.add_argument("--streamed-only", action="store_true",
                     help="run only the out-of-core (V in host memory) workload and print its line")
+    ap.add_argument("--size-sweep", action="store_true",
+                    help="run only the headline kernels on inputs of growing height (m = 1000, 2000) and print their rates")
     ap.add_argument("--shards-only", action="store_true",
                     help="run only one GPU's row block of an 8-way split (dense rank and headline shapes)")
     ap.add_argument("--dense-only", action="store_true",
@@ -307,6 +331,12 @@ def main():
         print(json.dumps({"metric": "dense rank: factorizations/s (BF16 contractions)",
                           "value": d["bf16_factorizations_per_s"], "unit": "factorizations/s", "n_gpus": 1,
                           "higher_is_better": True, "dense_rank": d}))
+        return
+    if args.size_sweep:
+        r = bench_size_sweep(pkg)
+        best = max(max(x["GBps_H"], x["GBps_W"]) for x in r)
+        print(json.dumps({"metric": "k = 16 partial kernel: GB/s by input height", "value": best, "unit": "GB/s",
+                          "n_gpus": 1, "higher_is_better": True, "size_sweep": r}))
         return
     if args.shards_only:
         r = bench_row_block_shards(pkg)

@@ -46,6 +46,19 @@ def load_package():
     return mod
 
 
+def host_cpu() -> dict:
+    """The host the CPU baselines ran on: logical CPUs and the processor model."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "model": model}
+
+
 def cpu_baseline(sample_iters: int) -> dict:
     """Oracle's FP32 scalar rendering of the reference update sequence on the host cores
     (TEST INFRASTRUCTURE, timed as a reported baseline only)."""
@@ -60,7 +73,7 @@ def cpu_baseline(sample_iters: int) -> dict:
     r = json.loads(out.strip().splitlines()[-1])
     s_iter = r["s_per_iter"]
     return {"value": 1.0 / (ITERS * s_iter), "unit": "factorizations/s", "cores": r["threads"],
-            "kind": "port",
+            "kind": "port", "host": host_cpu(),
             "sample": f"{sample_iters} iterations of the {N}x{M} k={K} FP32 update loop "
                       f"(s/iter {s_iter:.4f}) extrapolated to {ITERS} iterations"}
 

@@ -258,7 +258,7 @@ typedef struct nmfk_serve_config {
 	uint32_t chunks_per_poll;
 	uint32_t plan_target;         /* partial-kernel workgroups each job's plan aims for (0: the
 	                                 single-job tunables); many jobs share the device, so fewer
-	                                 splits per job than alone (default 32) */
+	                                 splits per job than alone (default 64) */
 	uint32_t progress_every;      /* print a progress line to stderr every N retired requests (0: never) */
 	double mu, sigma;             /* V = exp(N(mu, sigma^2)) */
 } nmfk_serve_config;

@@ -3,6 +3,7 @@ bionmf-gpu_amd package (its directory name has a hyphen, so it is loaded by path
 from __future__ import annotations
 
 import importlib.util
+import os
 import pathlib
 import sys
 
@@ -36,10 +37,17 @@ def pkg():
 
 @pytest.fixture(scope="session")
 def nmfk(pkg):
-    """The loaded product library on a GPU box; the HIP path must be the one that runs."""
+    """The loaded product library on a GPU box; the HIP path must be the one that runs.
+
+    NMFK_TEST_TUNE="key=value,..." (planner tunables, nmfk_tune_set keys) runs the session
+    under other launch shapes, e.g. "4=2,5=2" for 2 x 2 matrix-core workgroups.  Batching
+    takes the default shapes only, so such sessions run the single-job test files."""
     if pkg.gpu_count() < 1:
         pytest.skip("no HIP device visible")
     pkg.load()  # raises if libnmfgpu_amd.so is missing: no silent fallback
+    for item in filter(None, os.environ.get("NMFK_TEST_TUNE", "").split(",")):
+        key, value = (int(x) for x in item.split("="))
+        pkg.tune_set(key, value)
     return pkg
 
 

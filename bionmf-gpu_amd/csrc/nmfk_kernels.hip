@@ -1784,10 +1784,11 @@ static inline uint32_t default_ap(uint32_t kt)
 
 // Launch-shape tunables (workgroups the partial kernels aim for, minimum steps per wave,
 // most splits, and the MFMA workgroup shape NA x NB).
-// The BF16 kernel (kt 64) runs 2 waves per SIMD: 512 four-wave workgroups fill the 256
-// CUs once.  At 50000 x 2000, k = 64 its H half takes 105 us with the 16 splits that gives
-// against 115 us with the 32 of the FP32 target.
-static int g_tune[NMFK_TUNE_COUNT] = {512, 1024, 8, 64, 1, 4, 512};
+// The BF16 kernel (kt 64) reads at the memory system's rate for its access pattern with
+// one four-wave workgroup per CU as well as with two.  At 50000 x 2000, k = 64 its H half
+// takes 101 us with the 8 splits a 256-workgroup target gives, 105 us with 16 (512) and
+// 115 us with the 32 of the FP32 target; 12 splits (384) take 127 us.
+static int g_tune[NMFK_TUNE_COUNT] = {512, 1024, 8, 64, 1, 4, 256};
 
 static bool mfma_shape_ok(int na, int nb)
 {

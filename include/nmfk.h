@@ -111,7 +111,7 @@ int nmfk_job_kernel_path(const nmfk_job *job);
  * splits of the contraction range (default 64), 4 and 5 = matrix-core workgroup shape:
  * waves side by side along the updated factor x wave groups splitting the contraction
  * range (default 1 x 4; also 2x2, 4x1, 2x4, 4x2), 6 = workgroups the BF16 matrix-core
- * partial kernel aims for (default 512). */
+ * partial kernel aims for (default 256). */
 int nmfk_tune_set(int key, int value);
 int nmfk_tune_get(int key);
 

@@ -7,7 +7,7 @@ import ctypes as C
 import pytest
 
 PATH_AUTO, PATH_VALU, PATH_MFMA32, PATH_BF16 = 0, 1, 2, 3
-DEFAULTS = {0: 512, 1: 1024, 2: 8, 3: 64, 4: 1, 5: 4, 6: 512}
+DEFAULTS = {0: 512, 1: 1024, 2: 8, 3: 64, 4: 1, 5: 4, 6: 256}
 
 
 class HPlan(C.Structure):
@@ -77,11 +77,11 @@ def test_plan_covers_both_ranges(lib, p, q, kt, path):
 
 
 def test_bf16_plans_of_the_dense_rank_workload(lib):
-    # V 50000 x 2000, k = 64: the H half (rows of H against the 50000 rows of V) in 16 splits
-    # of 3136 rows, 33 x 16 = 528 workgroups (with the denominator column); the W half in one
+    # V 50000 x 2000, k = 64: the H half (rows of H against the 50000 rows of V) in 8 splits
+    # of 6272 rows, 33 x 8 = 264 workgroups (with the denominator column); the W half in one
     # split over the 2000 columns.
     h, w = plan(lib, 2000, 50000, 64), plan(lib, 50000, 2000, 64)
-    assert (h.path, h.nab, h.S, h.bchunk) == (PATH_BF16, 32, 16, 3136)
+    assert (h.path, h.nab, h.S, h.bchunk) == (PATH_BF16, 32, 8, 6272)
     assert (w.path, w.nab, w.S, w.bchunk) == (PATH_BF16, 782, 1, 2016)
     # the FP32 matrix-core kernel keeps its own target of 1024 workgroups
     assert plan(lib, 2000, 50000, 64, PATH_MFMA32).S == 32
@@ -98,8 +98,8 @@ def test_tunables_move_the_plan(lib):
     assert (plan(lib, 1000, 20000, 16).S, plan(lib, 1000, 20000, 16).bchunk) == (32, 640)
     assert lib.nmfk_tune_set(2, 1) == 0 and lib.nmfk_tune_set(1, 2 * DEFAULTS[1]) == 0
     assert (plan(lib, 1000, 20000, 16).S, plan(lib, 1000, 20000, 16).bchunk) == (63, 320)
-    assert lib.nmfk_tune_set(6, 256) == 0
-    assert plan(lib, 2000, 50000, 64).S == 8
+    assert lib.nmfk_tune_set(6, 512) == 0
+    assert plan(lib, 2000, 50000, 64).S == 16
     assert lib.nmfk_tune_set(4, 2) == 0 and lib.nmfk_tune_set(5, 2) == 0
     pl = plan(lib, 1000, 20000, 16)
     assert (pl.ap, pl.nw, pl.nab) == (2, 4, 8)

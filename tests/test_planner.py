@@ -70,7 +70,9 @@ def test_plan_covers_both_ranges(lib, p, q, kt, path):
         assert pl.lb == 8 or p > 64 * pl.ap // (2 * pl.lb)
     else:
         step, rows_a = (32 if path == PATH_BF16 else 16), 64 * pl.ap
-        assert (pl.ap, pl.nw) == (1, 4) and pl.lb == 1
+        # matrix cores: 1 x 4 waves, or 2 x 2 for BF16 over fewer than 4096 rows of U
+        shape = (2, 4) if path == PATH_BF16 and q < 4096 else (1, 4)
+        assert (pl.ap, pl.nw) == shape and pl.lb == 1
     assert pl.nab == -(-p // rows_a)
     assert pl.bchunk % step == 0 and pl.S >= 1 and pl.S <= DEFAULTS[3]
     assert (pl.S - 1) * pl.bchunk < q <= pl.S * pl.bchunk   # every split holds rows, together all q
@@ -78,11 +80,11 @@ def test_plan_covers_both_ranges(lib, p, q, kt, path):
 
 def test_bf16_plans_of_the_dense_rank_workload(lib):
     # V 50000 x 2000, k = 64: the H half (rows of H against the 50000 rows of V) in 8 splits
-    # of 6272 rows, 33 x 8 = 264 workgroups (with the denominator column); the W half in one
-    # split over the 2000 columns.
+    # of 6272 rows, 33 x 8 = 264 workgroups (with the denominator column), 1 x 4 waves; the W
+    # half in one split over the 2000 columns, 2 x 2 waves (128 rows of W per workgroup).
     h, w = plan(lib, 2000, 50000, 64), plan(lib, 50000, 2000, 64)
-    assert (h.path, h.nab, h.S, h.bchunk) == (PATH_BF16, 32, 8, 6272)
-    assert (w.path, w.nab, w.S, w.bchunk) == (PATH_BF16, 782, 1, 2016)
+    assert (h.path, h.nab, h.S, h.bchunk, h.ap, h.nw) == (PATH_BF16, 32, 8, 6272, 1, 4)
+    assert (w.path, w.nab, w.S, w.bchunk, w.ap, w.nw) == (PATH_BF16, 391, 1, 2016, 2, 4)
     # the FP32 matrix-core kernel keeps its own target of 1024 workgroups
     assert plan(lib, 2000, 50000, 64, PATH_MFMA32).S == 32
 

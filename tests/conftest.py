@@ -37,7 +37,7 @@ def pkg():
 
 @pytest.fixture(scope="session")
 def nmfk(pkg):
-    """The loaded product library on a GPU box; the HIP path must be the one that runs.
+    """The loaded product library on a machine with an MI355X; the HIP path must be the one that runs.
 
     NMFK_TEST_TUNE="key=value,..." (planner tunables, nmfk_tune_set keys) runs the session
     under other launch shapes, e.g. "4=2,5=2" for 2 x 2 matrix-core workgroups.  Batching

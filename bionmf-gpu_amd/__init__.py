@@ -164,6 +164,8 @@ def load(path: os.PathLike | str | None = None):
     _sig(lib, "nmfk_job_get_W", i32, [vp, _F32P, sz])
     _sig(lib, "nmfk_job_get_H", i32, [vp, _F32P, sz])
     _sig(lib, "nmfk_job_run", i32, [vp, C.POINTER(Options), C.POINTER(Result)])
+    _sig(lib, "nmfk_job_run_traced", i32, [vp, C.POINTER(Options), C.POINTER(Result), C.c_void_p, C.c_uint32,
+                                           C.POINTER(C.c_uint32)])
     _sig(lib, "nmfk_job_update_H", i32, [vp])
     _sig(lib, "nmfk_job_update_W", i32, [vp])
     _sig(lib, "nmfk_job_adjust", i32, [vp])
@@ -425,11 +427,22 @@ class Job:
         return a.value, b.value, c.value
 
     # the loop -----------------------------------------------------------
-    def run(self, nIters=2000, niter_test_conv=10, stop_threshold=40) -> dict:
+    def run(self, nIters=2000, niter_test_conv=10, stop_threshold=40, trace=False) -> dict:
+        """nmfk_job_run; with trace=True also 'diff_trace', the connectivity difference the
+        device counted at every convergence test (nmfk_job_run_traced)."""
         opt = Options(self.k, nIters, niter_test_conv, stop_threshold)
         res = Result()
-        _check(self.lib.nmfk_job_run(self.h, C.byref(opt), C.byref(res)), "run")
-        return res.as_dict()
+        if not trace:
+            _check(self.lib.nmfk_job_run(self.h, C.byref(opt), C.byref(res)), "run")
+            return res.as_dict()
+        cap = nIters // max(niter_test_conv, 1) + 1
+        buf = np.zeros(cap, np.uint64)
+        nt = C.c_uint32()
+        _check(self.lib.nmfk_job_run_traced(self.h, C.byref(opt), C.byref(res), buf.ctypes.data, cap,
+                                            C.byref(nt)), "run")
+        out = res.as_dict()
+        out["diff_trace"] = buf[:min(nt.value, cap)].copy()
+        return out
 
     def update_H(self):
         _check(self.lib.nmfk_job_update_H(self.h), "update_H")

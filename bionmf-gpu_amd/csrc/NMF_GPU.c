@@ -115,7 +115,7 @@ int main(int argc, char *argv[])
 
 	bool help = false;
 	struct input_arguments a;
-	if (check_arguments(argc, (char const *const *)argv, &help, &a) != EXIT_SUCCESS)
+	if (check_arguments(argc, (char const *restrict *restrict)argv, &help, &a) != EXIT_SUCCESS)
 		return EXIT_FAILURE;
 	if (help)
 		return print_nmf_gpu_help(argv[0]);
@@ -190,11 +190,8 @@ int main(int argc, char *argv[])
 	/* the reference's precision: FP32 arithmetic at every rank (streamed jobs always are) */
 	if (!streamed && nmfk_job_kernel_path(job) == 3)
 		nmfk_job_set_kernel_path(job, 2);
-	/* NMFGPU_SEED (environment) overrides the seed, for reproducible runs. */
-	index_t seed = get_seed();
-	const char *env_seed = getenv("NMFGPU_SEED");
-	if (env_seed && *env_seed)
-		seed = (index_t)strtoul(env_seed, NULL, 10);
+	/* get_seed() returns NMFGPU_SEED from the environment when set, for reproducible runs. */
+	const index_t seed = get_seed();
 	if ((!streamed && nmfk_job_set_V(job, V, m)) || nmfk_job_init_random(job, seed)) {
 		print_error(true, "Error initializing the factorization: %s\n", nmfk_last_error());
 		goto done;

@@ -175,7 +175,8 @@ static int parse_index(char const *s, intmax_t lo, intmax_t hi, index_t *out)
 	return 0;
 }
 
-int check_arguments(int argc, char const *const *argv, bool *help_requested, struct input_arguments *arguments)
+int check_arguments(int argc, char const *restrict *restrict argv, bool *restrict help_requested,
+		    struct input_arguments *restrict arguments)
 {
 	if (argc <= 0 || !argv || !help_requested || !arguments) {
 		print_errnum(false, EFAULT, "check_arguments");
@@ -346,8 +347,15 @@ size_t get_difference(index_t const *cls, index_t const *last, index_t m)
 
 /* ---------------------------------------------------------------- seed */
 
+/* src/common.c:1218-1258: FIXED_SEED in a NMFGPU_FIXED_INIT build, else /dev/urandom (the
+ * clock if that fails).  Addition: a decimal NMFGPU_SEED in the environment is returned
+ * instead, so that a driver built on this library (the reference's own bin/NMF_GPU included)
+ * can be run reproducibly. */
 index_t get_seed(void)
 {
+	const char *env = getenv("NMFGPU_SEED");
+	if (env && *env)
+		return (index_t)strtoul(env, NULL, 10);
 	index_t seed = FIXED_SEED;
 #ifndef NMFGPU_FIXED_INIT
 	FILE *f = fopen("/dev/urandom", "r");

@@ -743,6 +743,7 @@ int nmfk_job_run(nmfk_job *j, const nmfk_options *opt, nmfk_result *res)
 			e = hipEventSynchronize(j->ev[(c - 1) & 1]);
 			if (e)
 				return nmfk_hip_fail(e, "interval wait");
+			nmfk_trace_state(j, &j->h_st[(c - 1) & 1]);
 			if (j->h_st[(c - 1) & 1].converged) {
 				converged = 1;
 				break;
@@ -765,6 +766,7 @@ int nmfk_job_run(nmfk_job *j, const nmfk_options *opt, nmfk_result *res)
 		return nmfk_hip_fail(e, "run");
 	const double t1 = nmfk_now();
 	const nmfk_state s = j->h_st[0];
+	nmfk_trace_state(j, &s);
 	j->colW_valid = 1;
 	if (res) {
 		res->converged = s.converged == 1;
@@ -775,6 +777,26 @@ int nmfk_job_run(nmfk_job *j, const nmfk_options *opt, nmfk_result *res)
 		res->pad_ = 0;
 	}
 	return 0;
+}
+
+int nmfk_job_run_traced(nmfk_job *j, const nmfk_options *opt, nmfk_result *res, uint64_t *diffs, uint32_t cap,
+			uint32_t *ntrace)
+{
+	if (!diffs && cap) {
+		nmfk_set_error("nmfk_job_run_traced: no trace buffer for %u entries", cap);
+		return NMFK_EINVAL;
+	}
+	j->tracing = 1;
+	j->diff_trace = diffs;
+	j->trace_cap = diffs ? cap : 0;
+	j->trace_len = 0;
+	const int e = nmfk_job_run(j, opt, res);
+	if (ntrace)
+		*ntrace = j->trace_len;
+	j->tracing = 0;
+	j->diff_trace = NULL;
+	j->trace_cap = j->trace_len = 0;
+	return e;
 }
 
 /* ---------------------------------------------------------------- checks */

@@ -61,7 +61,22 @@ struct nmfk_job {
 	hipEvent_t up[2], done[2];     /* slot s: block uploaded / block's kernels finished */
 	double stream_bytes;           /* bytes of V copied to the device by the last run */
 	int slot0_written;             /* a caller wrote slot 0 on the job's stream (drop-in upload) */
+	/* Per-test connectivity differences of the current run (nmfk_job_run_traced): the host
+	 * copies the device state after every test interval and keeps its last_diff. */
+	int tracing;
+	uint64_t *diff_trace;
+	uint32_t trace_cap, trace_len;
 };
+
+/* Records the difference of the test a state copy reports, once per test. */
+static inline void nmfk_trace_state(nmfk_job *j, const nmfk_state *s)
+{
+	if (j->tracing && s->ntests == j->trace_len + 1) {
+		if (j->trace_len < j->trace_cap)
+			j->diff_trace[j->trace_len] = s->last_diff;
+		j->trace_len++;
+	}
+}
 
 /* Streamed jobs: the loop and the Frobenius/KL pass (nmfk_stream.c). */
 int nmfk_stream_run(nmfk_job *j, const nmfk_options *opt, nmfk_result *res);

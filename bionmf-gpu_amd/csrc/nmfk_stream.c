@@ -439,8 +439,11 @@ int nmfk_stream_run(nmfk_job *j, const nmfk_options *opt, nmfk_result *res)
 				e = hipStreamSynchronize(j->stream);
 			if (e)
 				e = nmfk_hip_fail(e, "test state");
-			else if (j->h_st[0].converged)
-				break;
+			else {
+				nmfk_trace_state(j, &j->h_st[0]);
+				if (j->h_st[0].converged)
+					break;
+			}
 		}
 	}
 	if (e)
@@ -452,6 +455,7 @@ int nmfk_stream_run(nmfk_job *j, const nmfk_options *opt, nmfk_result *res)
 		return nmfk_hip_fail(e, "streamed run");
 	const double t1 = nmfk_now();
 	const nmfk_state s = j->h_st[0];
+	nmfk_trace_state(j, &s);
 	j->colW_valid = 1;
 	if (res) {
 		res->converged = s.converged == 1;

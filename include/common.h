@@ -85,8 +85,8 @@ index_t get_padding(index_t dim);
 
 /* Parses argv with the reference's option string ":B:b:CcE:e:HhI:i:J:j:K:k:RrT:t:Z:z:"
  * (src/common.c:847-1119); help_requested is set for -h. */
-int check_arguments(int argc, char const *const *argv, bool *help_requested,
-		    struct input_arguments *arguments);
+int check_arguments(int argc, char const *restrict *restrict argv, bool *restrict help_requested,
+		    struct input_arguments *restrict arguments);
 
 size_t next_power_2(size_t x);
 size_t prev_power_2(size_t x);
@@ -96,7 +96,8 @@ size_t prev_power_2(size_t x);
 size_t get_difference(index_t const *classification, index_t const *last_classification, index_t m);
 
 /* Seed: FIXED_SEED when built with NMFGPU_FIXED_INIT, else read from /dev/urandom with the
- * clock as fallback (src/common.c:1218-1258). */
+ * clock as fallback (src/common.c:1218-1258).  A decimal NMFGPU_SEED in the environment
+ * takes precedence (an addition, for reproducible runs of any driver on this library). */
 index_t get_seed(void);
 
 #ifdef __cplusplus

@@ -83,6 +83,12 @@ int nmfk_job_get_H(nmfk_job *job, float *H, size_t ldh);
 /* The whole loop of src/NMF_GPU.c:522-795 on the device. */
 int nmfk_job_run(nmfk_job *job, const nmfk_options *opt, nmfk_result *res);
 
+/* nmfk_job_run, also returning the connectivity difference the device counted at every
+ * convergence test (the values src/NMF_GPU.c:677-708 feeds to the stop rule): diffs[t] is
+ * that of test t + 1, for t < min(tests performed, cap); *ntrace = tests observed. */
+int nmfk_job_run_traced(nmfk_job *job, const nmfk_options *opt, nmfk_result *res, uint64_t *diffs,
+			uint32_t cap, uint32_t *ntrace);
+
 /* Single half-steps with the reference's structure (column sum, then the update). */
 int nmfk_job_update_H(nmfk_job *job);
 int nmfk_job_update_W(nmfk_job *job);

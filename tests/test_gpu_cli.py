@@ -100,3 +100,39 @@ def test_driver_streams_V_from_host_memory(nmfk, orc, tmp_path):
     assert a[0] == b[0] and a[3] == b[3]
     assert np.allclose(b[1], a[1], rtol=1e-3, atol=1e-6 * a[1].max())
     assert np.allclose(b[2], a[2], rtol=1e-3, atol=1e-6 * a[2].max())
+
+
+REF_DRIVER = ROOT / "oracle" / "_ref" / "NMF_GPU_ref"
+
+
+def test_reference_driver_on_library(nmfk, orc, tmp_path):
+    # The reference's own src/NMF_GPU.c + matrix I/O, built against libnmfgpu_amd.so by the
+    # integration recipe (make -C oracle refdriver): its main() and nmf() loop drive the
+    # drop-in API.  Its outputs equal the job API's run from the same seed.
+    if not REF_DRIVER.exists():
+        pytest.skip("oracle/_ref/NMF_GPU_ref was not built (reference tree absent at build time)")
+    n, m, k = 400, 50, 3
+    V = orc.gen_lognormal(n, m, 29, 3.0, 1.0)
+    f = tmp_path / "data.txt"
+    nmfk.save_matrix(f, V, nmfk.FMT_TEXT)
+    Vf = nmfk.load_matrix(f)[0]
+    env = dict(os.environ, NMFGPU_SEED="7")
+    r = subprocess.run([str(REF_DRIVER), str(f), "-k", str(k), "-i", "600", "-j", "10", "-t", "20"],
+                       capture_output=True, text=True, env=env, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    W = nmfk.load_matrix(tmp_path / "data.txt_W.txt")[0]
+    H = nmfk.load_matrix(tmp_path / "data.txt_H.txt")[0]
+    assert W.shape == (n, k) and H.shape == (k, m)
+    with nmfk.Job(n, m, k) as j:
+        j.set_V(Vf)
+        j.init_random(7)
+        res = j.run(600, 10, 20)
+        W_ref, H_ref = j.get_W(), j.get_H()
+        dv, dvwh, _ = j.dot_kl()
+    # text output carries 6 significant digits (%g)
+    assert np.allclose(W, W_ref, rtol=1e-5, atol=0) and np.allclose(H, H_ref.T, rtol=1e-5, atol=0)
+    out = r.stdout + r.stderr
+    if res["converged"]:
+        assert f"converged in {res['iterations']} iterations" in out, out[-2000:]
+    dist = float(out.split("Distance between V and W*H:")[1].split()[0])
+    assert abs(dist - np.sqrt(dvwh / dv)) <= 1e-4 * dist

@@ -186,16 +186,40 @@ def test_dot_and_kl_match_oracle(nmfk, orc):
 def test_stop_rule_on_device_matches_oracle_trace(nmfk, orc):
     # Per-test differences counted on the device (contingency table) reproduce the
     # oracle's pairwise count through the whole run, including the reset cases.
+    # Compared test by test: src/common.c:1195-1209 is integer work, so every count must be
+    # equal, the non-zero ones included (a count that only kept the zero pattern would fail).
     n, m, k = 400, 60, 4
     V = orc.gen_lognormal(n, m, 12, 0.0, 1.0)
     W0, H0 = orc.init_WH(n, m, k, 13)
-    o = orc.nmf(V, W0, H0, 600, 3, 6, fp64=True)
+    o = orc.nmf(V, W0, H0, 600, 3, 6, fp64=True, max_trace=400)
     with nmfk.Job(n, m, k) as job:
         job.set_V(V)
         job.init_random(13)
-        r = job.run(600, 3, 6)
+        r = job.run(600, 3, 6, trace=True)
         assert r["iterations"] == o["iterations"] and r["ntests"] == o["ntests"]
         assert np.array_equal(job.get_classification(), o["cls"])
+        dev, ref = r["diff_trace"], o["diff_trace"]
+        assert len(dev) == r["ntests"] == len(ref)
+        assert np.array_equal(dev, ref), np.nonzero(dev != ref)
+        assert r["last_diff"] == ref[-1]
+        nz = np.count_nonzero(ref)
+        assert nz >= 3 and len(set(ref[ref > 0].tolist())) >= 2   # several distinct non-zero counts
+
+
+@pytest.mark.parametrize("n,m,k,seed", [(3000, 97, 3, 41), (1200, 300, 7, 42), (2500, 150, 12, 43)])
+def test_device_difference_counts_equal_oracle(nmfk, orc, n, m, k, seed):
+    # Larger m (more pairs, up to 44850) and every kernel path (VALU at k <= 8, matrix cores
+    # at k = 12): the device's per-test differences equal the FP64 oracle's pairwise counts.
+    V = orc.gen_lognormal(n, m, seed, 5.0, 1.5)
+    W0, H0 = orc.init_WH(n, m, k, seed + 100)
+    o = orc.nmf(V, W0, H0, 400, 10, 40, fp64=True, max_trace=100)
+    with nmfk.Job(n, m, k) as job:
+        job.set_V(V)
+        job.init_random(seed + 100)
+        r = job.run(400, 10, 40, trace=True)
+    assert r["ntests"] == o["ntests"] == 40
+    assert np.array_equal(r["diff_trace"], o["diff_trace"]), (r["diff_trace"], o["diff_trace"])
+    assert np.count_nonzero(o["diff_trace"]) >= 2
 
 
 def test_full_size_config1_shape(nmfk, orc):

@@ -151,6 +151,69 @@ def test_adjust_special_values(orc):
         assert np.array_equal(got.astype(np.float64), np.array(want, np.float64).astype(got.dtype).astype(np.float64))
 
 
+# ---------------------------------------------------------------- elementwise ops
+
+def _mul_div_inputs(dt):
+    # Column-dependent divisors on a non-square shape, so a row/column mix-up in acc[] fails.
+    rng = np.random.default_rng(31)
+    A = (rng.random((37, 5)) * 3 + 0.01).astype(dt)
+    Aux = (rng.random((37, 5)) * 7 + 0.01).astype(dt)
+    acc = np.array([0.3, 1.7, 3.1, 11.0, 0.07], dt)
+    return A, Aux, acc
+
+
+@pytest.mark.parametrize("fp64", [True, False])
+def test_mul_div_multiplies_then_divides(orc, fp64):
+    # src/GPU_kernels.cu:2029-2033: A = (A * Aux) / acc[col], one rounding per operation.
+    dt = np.float64 if fp64 else np.float32
+    A, Aux, acc = _mul_div_inputs(dt)
+    want = np.divide(np.multiply(A, Aux, dtype=dt), acc[None, :], dtype=dt)
+    got = orc.mul_div(A, Aux, acc, fp64=fp64)
+    assert got.dtype == dt and np.array_equal(got, want)
+    # The inputs tell the orders apart: dividing first rounds differently on some entries.
+    other1 = np.multiply(A, np.divide(Aux, acc[None, :], dtype=dt), dtype=dt)
+    other2 = np.multiply(np.divide(A, acc[None, :], dtype=dt), Aux, dtype=dt)
+    assert np.any(other1 != want) and np.any(other2 != want)
+    # and a row-indexed divisor would differ too
+    assert not np.array_equal(got, np.divide(np.multiply(A, Aux, dtype=dt)[:5, :].T, acc[None, :], dtype=dt).T)
+
+
+def test_mul_div_overflow_reveals_order(orc):
+    # In FP32 (A * Aux) overflows before the division; A * (Aux / acc) would not.
+    A = np.array([[1e30, 2.0]], np.float32)
+    Aux = np.array([[1e30, 3.0]], np.float32)
+    acc = np.array([1e30, 4.0], np.float32)
+    got = orc.mul_div(A, Aux, acc, fp64=False)
+    assert np.isposinf(got[0, 0]) and got[0, 1] == np.float32(1.5)
+    # FP64 has the range for the product: (a * a) / a with a = float(1e30f)
+    a = float(A[0, 0])
+    got64 = orc.mul_div(A.astype(np.float64), Aux.astype(np.float64), acc.astype(np.float64))
+    assert got64[0, 0] == (a * a) / a and got64[0, 1] == 1.5
+
+
+@pytest.mark.parametrize("fp64", [True, False])
+def test_div_sub_both_branches(orc, fp64):
+    # src/GPU_kernels.cu:1676-1700: div_operator != 0 -> A = B ./ A, else A = B - A.
+    dt = np.float64 if fp64 else np.float32
+    rng = np.random.default_rng(32)
+    A = (rng.random((23, 9)) * 5 + 0.1).astype(dt)
+    B = (rng.random((23, 9)) * 5 + 0.1).astype(dt)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        q = orc.div_sub(A, B, True, fp64=fp64)
+        assert np.array_equal(q, np.divide(B, A, dtype=dt))
+        assert not np.allclose(q, np.divide(A, B, dtype=dt))       # operands not swapped
+        d = orc.div_sub(A, B, False, fp64=fp64)
+        assert np.array_equal(d, np.subtract(B, A, dtype=dt))
+        assert np.any(np.sign(d) != np.sign(np.subtract(A, B, dtype=dt)))  # B - A, not A - B
+        # special values of the IEEE division and subtraction
+        Z = np.array([[0.0, 0.0, 2.0, np.inf]], dt)
+        Y = np.array([[3.0, 0.0, 2.0, 1.0]], dt)
+        q = orc.div_sub(Z, Y, True, fp64=fp64)[0]
+        assert np.isposinf(q[0]) and np.isnan(q[1]) and q[2] == 1.0 and q[3] == 0.0
+        d = orc.div_sub(Z, Y, False, fp64=fp64)[0]
+        assert d[0] == 3.0 and d[1] == 0.0 and d[2] == 0.0 and np.isneginf(d[3])
+
+
 # ---------------------------------------------------------------- updates
 
 def numpy_update_H(V, W, H):

@@ -173,9 +173,31 @@ def bench_sweep(pkg) -> dict:
             "autotune_seconds": a["tune_seconds"], "tuning": a["tuning"], "plan_target": a["plan_target"]}
 
 
+def fp64_errors(g, W, H, cls, kl, dv, dvwh) -> dict:
+    """Errors of a device run against the FP64 oracle's run of the same workload from the
+    same W0, H0 (tests/golden/config*_fp64.npz, written by tests/golden/make_golden.py)."""
+    step = int(g["params"][7])
+    rows = np.arange(0, W.shape[0], step)
+    WH = W[rows].astype(np.float64) @ H.T.astype(np.float64)
+    WHr = g["W_rows"].astype(np.float64) @ g["H"].T.astype(np.float64)
+    d_ref = np.sqrt(float(g["dot_VWH"]) / float(g["dot_V"]))
+    return {"rel_kl": abs(kl - float(g["kl"])) / float(g["kl"]),
+            "rel_frobenius_WH_sample": float(np.linalg.norm(WH - WHr) / np.linalg.norm(WHr)),
+            "WH_sample_rows": f"every {step}th row",
+            "rel_distance": abs(np.sqrt(dvwh / dv) - d_ref) / d_ref,
+            "class_agreement": float(np.mean(cls == g["cls"])),
+            "identical_class_vector": bool(np.array_equal(cls, g["cls"]))}
+
+
+def load_fixture(name):
+    f = ROOT / "tests" / "golden" / f"{name}.npz"
+    return np.load(f) if f.exists() else None
+
+
 def bench_dense_rank(pkg) -> dict:
     """Dense rank: V 50000 x 2000 planted rank 48, k = 64, 300 iterations: BF16 contractions
-    (FP32 accumulation) against the FP32 matrix-core path on the same start."""
+    (FP32 accumulation) against the FP32 matrix-core path on the same start, and both
+    against the FP64 oracle's run of the same workload."""
     n, m, k, iters = 50000, 2000, 64, 300
     job = pkg.Job(n, m, k)
     job.gen_V_planted(4848, 48, 0.3)
@@ -191,8 +213,14 @@ def bench_dense_rank(pkg) -> dict:
         dv, dvwh, kl = job.dot_kl()
         out[path] = {"seconds": dt, "ms_per_iteration": tm["ms_per_iter"],
                      "ms_partial": 0.5 * (tm["ms_partial_H"] + tm["ms_partial_W"]),
-                     "kl": kl, "W": job.get_W(), "H": job.get_H(), "cls": job.get_classification()}
+                     "kl": kl, "W": job.get_W(), "H": job.get_H(), "cls": job.get_classification(),
+                     "dv": dv, "dvwh": dvwh}
     job.close()
+    g = load_fixture("config3_fp64")
+    vs64 = None
+    if g is not None:
+        vs64 = {name: fp64_errors(g, o["W"], o["H"], o["cls"], o["kl"], o["dv"], o["dvwh"])
+                for name, o in (("bf16", out[3]), ("fp32", out[2]))}
     b, f = out[3], out[2]
     rows = np.arange(0, n, 7)
     WHb = b["W"][rows].astype(np.float64) @ b["H"].T.astype(np.float64)
@@ -205,7 +233,8 @@ def bench_dense_rank(pkg) -> dict:
             "fp32_partial_GBps": part_bytes / (f["ms_partial"] * 1e-3) / 1e9,
             "bf16_vs_fp32": {"rel_frobenius_WH_rows_step7": float(np.linalg.norm(WHb - WHf) / np.linalg.norm(WHf)),
                              "rel_kl_diff": abs(b["kl"] - f["kl"]) / f["kl"],
-                             "class_agreement": float(np.mean(b["cls"] == f["cls"]))}}
+                             "class_agreement": float(np.mean(b["cls"] == f["cls"]))},
+            "vs_fp64": vs64}
 
 
 def bench_row_block_shards(pkg, gpus: int = 8) -> dict:
@@ -394,6 +423,17 @@ def main():
     dv, dvwh, kl = job.dot_kl()
     value = args.steps / elapsed
 
+    # Accuracy of the measured path: one more factorization (untimed) from the FP64 fixture's
+    # starting point, compared with the oracle's FP64 run of the same workload.
+    accuracy = None
+    g1 = load_fixture("config1_fp64")
+    if g1 is not None and args.kernel_path == "auto":
+        job.init_random(int(g1["params"][5]))
+        ra = job.run(ITERS, T, 1_000_000_000)
+        adv, advwh, akl = job.dot_kl()
+        accuracy = fp64_errors(g1, job.get_W(), job.get_H(), job.get_classification(), akl, adv, advwh)
+        accuracy["iterations"] = ra["iterations"]
+
     # Dominant-kernel roofline.  Both half-steps launch the same partial (contraction)
     # kernel, and each launch reads V (or its transposed copy) once plus both factors, so
     # they share one algorithmic byte count; the average launch duration comes from HIP
@@ -451,6 +491,7 @@ def main():
         "iteration_GBps": iter_bytes / (tm["ms_per_iter"] * 1e-3) / 1e9,
         "kl_final": kl,
         "frobenius_ratio": float(np.sqrt(dvwh / dv)),
+        "vs_fp64": accuracy,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": ("k_partial_mfma<16,1,4>" if args.kernel_path in ("auto", "mfma32")

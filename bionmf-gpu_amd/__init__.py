@@ -102,6 +102,12 @@ class RequestResult(C.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class RestartResult(C.Structure):
+    _fields_ = [("iterations", C.c_uint32), ("converged", C.c_uint32), ("ntests", C.c_uint32), ("pad_", C.c_uint32),
+                ("last_diff", C.c_uint64), ("kl", C.c_double), ("cls", C.c_void_p), ("W", C.c_void_p),
+                ("H", C.c_void_p)]
+
+
 class ServeConfig(C.Structure):
     _fields_ = [("hbm_budget", C.c_size_t), ("capacity", C.c_uint32), ("nIters", C.c_uint32),
                 ("niter_test_conv", C.c_uint32), ("stop_threshold", C.c_uint32),
@@ -209,6 +215,9 @@ def load(path: os.PathLike | str | None = None):
     _sig(lib, "nmfk_request_bytes", sz, [u32, u32, u32, u32])
     _sig(lib, "nmfk_serve_run", i32, [C.POINTER(ServeConfig), C.POINTER(Request), u32,
                                       C.POINTER(RequestResult), C.POINTER(ServeStats)])
+    _sig(lib, "nmfk_serve_run_detail", i32, [C.POINTER(ServeConfig), C.POINTER(Request), u32,
+                                             C.POINTER(RequestResult), C.POINTER(ServeStats),
+                                             C.POINTER(RestartResult)])
     _lib = lib
     return lib
 
@@ -605,9 +614,13 @@ def sweep(src: Job, ks, restarts: int, base_seed: int, nIters=2000, niter_test_c
 
 
 def serve(requests, hbm_budget=200 << 30, capacity=1024, nIters=2000, niter_test_conv=10,
-          stop_threshold=40, mu=5.0, sigma=1.5, chunks_per_poll=1, plan_target=None, progress_every=0):
+          stop_threshold=40, mu=5.0, sigma=1.5, chunks_per_poll=1, plan_target=None, progress_every=0,
+          detail=False, factors_of=()):
     """nmfk_serve_run over [(n, m, k, restarts, seed, arrival_s), ...] (arrivals sorted).
-    Returns (per-request result dicts, stats dict)."""
+    Returns (per-request result dicts, stats dict).  With detail=True (nmfk_serve_run_detail)
+    each request dict also has 'restarts': per-restart dicts with iterations, converged,
+    ntests, last_diff, kl and the final class vector 'cls'; requests whose index is in
+    factors_of also get the final 'W' and 'H' of every restart."""
     lib = load()
     cfg = ServeConfig()
     lib.nmfk_serve_default_config(C.byref(cfg))
@@ -623,8 +636,37 @@ def serve(requests, hbm_budget=200 << 30, capacity=1024, nIters=2000, niter_test
         req[i] = Request(n, m, k, r, seed, t)
     out = (RequestResult * max(nr, 1))()
     st = ServeStats()
-    _check(lib.nmfk_serve_run(C.byref(cfg), req, nr, out, C.byref(st)), "serve")
-    return [out[i].as_dict() for i in range(nr)], st.as_dict()
+    if not detail:
+        _check(lib.nmfk_serve_run(C.byref(cfg), req, nr, out, C.byref(st)), "serve")
+        return [out[i].as_dict() for i in range(nr)], st.as_dict()
+    total = sum(q[3] for q in requests)
+    recs = (RestartResult * max(total, 1))()
+    bufs = []
+    o = 0
+    for i, (n, m, k, r, _, _) in enumerate(requests):
+        for _ in range(r):
+            cls = np.zeros(m, np.uint32)
+            W = np.zeros((n, k), np.float32) if i in factors_of else None
+            H = np.zeros((m, k), np.float32) if i in factors_of else None
+            recs[o].cls = cls.ctypes.data
+            recs[o].W = W.ctypes.data if W is not None else None
+            recs[o].H = H.ctypes.data if H is not None else None
+            bufs.append((cls, W, H))
+            o += 1
+    _check(lib.nmfk_serve_run_detail(C.byref(cfg), req, nr, out, C.byref(st), recs), "serve")
+    res = [out[i].as_dict() for i in range(nr)]
+    o = 0
+    for i, q in enumerate(requests):
+        res[i]["restarts"] = []
+        for _ in range(q[3]):
+            cls, W, H = bufs[o]
+            d = {f: getattr(recs[o], f) for f in ("iterations", "converged", "ntests", "last_diff", "kl")}
+            d["cls"] = cls
+            if W is not None:
+                d["W"], d["H"] = W, H
+            res[i]["restarts"].append(d)
+            o += 1
+    return res, st.as_dict()
 
 
 def request_stream(count, rate, seed, n_range=(1000, 60000), m_range=(20, 3000), k_range=(2, 32),

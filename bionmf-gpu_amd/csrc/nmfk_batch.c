@@ -43,7 +43,13 @@ struct nmfk_batch {
 	double *t_add;
 	uint32_t *slot_tiles;         /* tiles a slot holds in the current tables */
 	uint8_t *fresh;               /* added after the last launched chunk: its gathered state is not yet valid */
+	/* Job descriptors: h_desc is the host's working copy, rewritten by add and poll at any
+	 * time; uploads go through two pinned staging copies, each rewritten only after its
+	 * previous upload has completed (up_ev), so no buffer is ever changed under a copy. */
 	nmfk_bjob *h_desc, *d_desc;
+	nmfk_bjob *h_up[2];
+	hipEvent_t up_ev[2];
+	int up_pending[2], up_next;
 	nmfk_state *d_states, *h_states;
 	nmfk_tile *h_tiles, *d_tiles;
 	size_t tiles_cap, dtiles_cap;
@@ -98,6 +104,12 @@ nmfk_batch *nmfk_batch_create(uint32_t niter_test_conv, uint32_t cap)
 			  "hipHostMalloc(batch states)");
 	if (!e) e = hipck(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate");
 	if (!e) e = hipck(hipEventCreateWithFlags(&b->ev, hipEventDisableTiming), "hipEventCreate");
+	for (int i = 0; i < 2 && !e; i++) {
+		e = hipck(hipHostMalloc((void **)&b->h_up[i], cap * sizeof(nmfk_bjob), hipHostMallocDefault),
+			  "hipHostMalloc(batch job staging)");
+		if (!e)
+			e = hipck(hipEventCreateWithFlags(&b->up_ev[i], hipEventDisableTiming), "hipEventCreate");
+	}
 	if (!e) e = hipck(hipMemsetAsync(b->d_desc, 0, cap * sizeof(nmfk_bjob), b->stream), "clear batch");
 	if (!e) e = hipck(hipStreamSynchronize(b->stream), "clear batch");
 	if (e) {
@@ -123,6 +135,12 @@ void nmfk_batch_destroy(nmfk_batch *b)
 	hipFree(b->d_tiles);
 	if (b->h_states)
 		hipHostFree(b->h_states);
+	for (int i = 0; i < 2; i++) {
+		if (b->h_up[i])
+			hipHostFree(b->h_up[i]);
+		if (b->up_ev[i])
+			hipEventDestroy(b->up_ev[i]);
+	}
 	if (b->ev)
 		hipEventDestroy(b->ev);
 	if (b->stream)
@@ -218,6 +236,30 @@ int nmfk_batch_add(nmfk_batch *b, nmfk_job *j, const nmfk_options *opt)
 	b->tables_dirty = 1;
 	b->desc_dirty = 1;
 	return (int)s;
+}
+
+/* Uploads descriptors [0, count) to the device, ordered on the batch stream, through the
+ * staging copy whose last upload has finished. */
+static int upload_desc(nmfk_batch *b, uint32_t count)
+{
+	if (count == 0)
+		return 0;
+	const int i = b->up_next;
+	if (b->up_pending[i]) {
+		int e = hipck(hipEventSynchronize(b->up_ev[i]), "batch job upload wait");
+		if (e)
+			return e;
+		b->up_pending[i] = 0;
+	}
+	memcpy(b->h_up[i], b->h_desc, count * sizeof(nmfk_bjob));
+	int e = hipMemcpyAsync(b->d_desc, b->h_up[i], count * sizeof(nmfk_bjob), hipMemcpyHostToDevice, b->stream);
+	if (!e)
+		e = hipEventRecord(b->up_ev[i], b->stream);
+	if (e)
+		return hipck(e, "upload jobs");
+	b->up_pending[i] = 1;
+	b->up_next = i ^ 1;
+	return 0;
 }
 
 /* ---------------------------------------------------------------- tile tables */
@@ -328,20 +370,22 @@ static int build_tables(nmfk_batch *b)
 	 * be replaced in place. */
 	int e = 0;
 	if (n > b->dtiles_cap) {
+		/* grow by half again, so that a growing batch reallocates rarely */
+		const size_t want = (size_t)n + n / 2;
 		hipFree(b->d_tiles);
 		b->d_tiles = NULL;
 		b->dtiles_cap = 0;
-		e = hipck(hipMalloc((void **)&b->d_tiles, (size_t)n * sizeof(nmfk_tile)), "hipMalloc(tiles)");
+		e = hipck(hipMalloc((void **)&b->d_tiles, want * sizeof(nmfk_tile)), "hipMalloc(tiles)");
 		if (e)
 			return e;
-		b->dtiles_cap = n;
+		b->dtiles_cap = want;
 	}
 	if (n)
 		e = hipck(hipMemcpyAsync(b->d_tiles, b->h_tiles, (size_t)n * sizeof(nmfk_tile), hipMemcpyHostToDevice,
 					 b->stream), "upload tiles");
 	if (!e)
-		e = hipck(hipMemcpyAsync(b->d_desc, b->h_desc, b->cap * sizeof(nmfk_bjob), hipMemcpyHostToDevice,
-					 b->stream), "upload jobs");
+		e = upload_desc(b, b->cap);
+	/* the pageable tile table is read by the copy above: wait for it before returning */
 	if (!e)
 		e = hipck(hipStreamSynchronize(b->stream), "upload tables");
 	if (e)
@@ -418,8 +462,7 @@ int nmfk_batch_step(nmfk_batch *b, uint32_t chunks)
 		if (!e)
 			e = build_tables(b);
 	} else if (b->desc_dirty) {
-		e = hipck(hipMemcpyAsync(b->d_desc, b->h_desc, b->nslots * sizeof(nmfk_bjob), hipMemcpyHostToDevice,
-					 b->stream), "upload jobs");
+		e = upload_desc(b, b->nslots);
 		b->desc_dirty = 0;
 	}
 	for (uint32_t c = 0; c < chunks && !e; c++)

@@ -108,21 +108,75 @@ static void layout_of(uint32_t n, uint32_t m, uint32_t k, job_layout *L)
 	L->n_dot = (size_t)nmfk_dot_kl_blocks(n, m) * 3;
 }
 
+/* Device bytes of a job's buffers, each rounded as the serving arena places it; with_V = 0
+ * for a job sharing another job's V and Vt. */
+static size_t layout_bytes(const job_layout *L, uint32_t m, int with_V)
+{
+	const size_t sz[] = {4 * L->n_W, 4 * L->n_H, 4 * L->n_part, 4 * L->n_colW, 4 * L->n_colH, 4 * L->n_den,
+			     8 * L->n_dot, 4 * (size_t)m, 4 * (size_t)m, sizeof(nmfk_state)};
+	size_t b = 0;
+	for (size_t i = 0; i < sizeof sz / sizeof sz[0]; i++)
+		b += nmfk_arena_round(sz[i]);
+	if (L->n_shadow)
+		b += nmfk_arena_round(2 * L->n_shadow);
+	if (with_V)
+		b += nmfk_arena_round(4 * L->n_V) + nmfk_arena_round(4 * L->n_Vt);
+	return b;
+}
+
 size_t nmfk_job_bytes(uint32_t n, uint32_t m, uint32_t k)
 {
 	job_layout L;
 	layout_of(n, m, k, &L);
-	size_t b = 4 * (L.n_V + L.n_Vt + L.n_W + L.n_H + L.n_part + L.n_colW + L.n_colH + L.n_den) + 2 * L.n_shadow;
-	b += 8 * L.n_dot + 4 * 2 * (size_t)m + sizeof(nmfk_state) + 7 * 256;
-	return b;
+	return layout_bytes(&L, m, 1);
 }
 
-static int dalloc(void **p, size_t bytes, const char *what)
+size_t nmfk_job_bytes_shared(uint32_t n, uint32_t m, uint32_t k)
+{
+	job_layout L;
+	layout_of(n, m, k, &L);
+	return layout_bytes(&L, m, 0);
+}
+
+/* A device buffer of job j: from its arena when it has one, else hipMalloc. */
+static int dalloc_job(nmfk_job *j, void **p, size_t bytes, const char *what)
 {
 	*p = NULL;
 	if (bytes == 0)
 		bytes = 4;
+	if (j->arena) {
+		*p = nmfk_arena_alloc(j->arena, bytes);
+		if (!*p) {
+			nmfk_set_error("%s: %zu bytes do not fit in the device arena", what, bytes);
+			return NMFK_EBUDGET;
+		}
+		return 0;
+	}
 	return nmfk_hip_fail(hipMalloc(p, bytes), what);
+}
+
+static void dfree_job(nmfk_job *j, void *p)
+{
+	if (!p)
+		return;
+	if (j->arena)
+		nmfk_arena_free(j->arena, p);
+	else
+		hipFree(p);
+}
+
+/* The pinned state slots and events nmfk_job_run uses; arena jobs (the serving loop's, which
+ * run batched) get them on first use. */
+static int run_resources(nmfk_job *j)
+{
+	int e = 0;
+	if (!j->h_st)
+		e = nmfk_hip_fail(hipHostMalloc((void **)&j->h_st, 2 * sizeof(nmfk_state), hipHostMallocDefault),
+				  "hipHostMalloc(state)");
+	for (int i = 0; i < 2 && !e; i++)
+		if (!j->ev[i])
+			e = nmfk_hip_fail(hipEventCreateWithFlags(&j->ev[i], hipEventDisableTiming), "hipEventCreate");
+	return e;
 }
 
 /* Jobs whose device V can be written: not shared (nmfk_job_create_shared), not streamed. */
@@ -230,11 +284,24 @@ static int d2h_rows(float *h, size_t hpitch, const float *d, size_t dpitch, size
 	return nmfk_hip_fail(e, "device-to-host copy");
 }
 
-static nmfk_job *job_create(uint32_t n, uint32_t m, uint32_t k, nmfk_job *src);
+static nmfk_job *job_create(uint32_t n, uint32_t m, uint32_t k, nmfk_job *src, nmfk_arena *arena,
+			    hipStream_t stream);
 
 nmfk_job *nmfk_job_create(uint32_t n, uint32_t m, uint32_t k)
 {
-	return job_create(n, m, k, NULL);
+	return job_create(n, m, k, NULL, NULL, NULL);
+}
+
+nmfk_job *nmfk_job_create_in(nmfk_arena *arena, hipStream_t stream, uint32_t n, uint32_t m, uint32_t k,
+			     nmfk_job *src)
+{
+	if (!arena || !stream) {
+		nmfk_set_error("nmfk_job_create_in: needs an arena and a stream");
+		return NULL;
+	}
+	if (src && (src->streamed || nmfk_job_refresh_Vt(src) || nmfk_job_sync(src)))
+		return NULL;
+	return job_create(n, m, k, src, arena, stream);
 }
 
 nmfk_job *nmfk_job_create_shared(nmfk_job *src, uint32_t k)
@@ -249,10 +316,11 @@ nmfk_job *nmfk_job_create_shared(nmfk_job *src, uint32_t k)
 	}
 	if (nmfk_job_refresh_Vt(src) || nmfk_job_sync(src))
 		return NULL;
-	return job_create(src->n, src->m, k, src);
+	return job_create(src->n, src->m, k, src, NULL, NULL);
 }
 
-static nmfk_job *job_create(uint32_t n, uint32_t m, uint32_t k, nmfk_job *src)
+static nmfk_job *job_create(uint32_t n, uint32_t m, uint32_t k, nmfk_job *src, nmfk_arena *arena,
+			    hipStream_t stream)
 {
 	if (n < 1 || m < 1 || k < 1 || k > NMFK_KMAX) {
 		nmfk_set_error("nmfk_job_create: invalid shape n=%u m=%u k=%u (1 <= k <= %d)", n, m, k, NMFK_KMAX);
@@ -270,6 +338,7 @@ static nmfk_job *job_create(uint32_t n, uint32_t m, uint32_t k, nmfk_job *src)
 	}
 	job_layout L;
 	layout_of(n, m, k, &L);
+	j->arena = arena;
 	j->n = n, j->m = m, j->k = k, j->kt = L.kt;
 	j->ldv = L.ldv, j->ldvt = L.ldvt;
 	j->planH = L.planH, j->planW = L.planW;
@@ -283,34 +352,36 @@ static nmfk_job *job_create(uint32_t n, uint32_t m, uint32_t k, nmfk_job *src)
 		j->V = src->V;
 		j->Vt = src->Vt;
 	} else {
-		e |= dalloc((void **)&j->V, 4 * L.n_V, "hipMalloc(V)");
-		if (!e) e |= dalloc((void **)&j->Vt, 4 * L.n_Vt, "hipMalloc(Vt)");
+		e |= dalloc_job(j, (void **)&j->V, 4 * L.n_V, "hipMalloc(V)");
+		if (!e) e |= dalloc_job(j, (void **)&j->Vt, 4 * L.n_Vt, "hipMalloc(Vt)");
 	}
-	if (!e) e |= dalloc((void **)&j->W, 4 * L.n_W, "hipMalloc(W)");
-	if (!e) e |= dalloc((void **)&j->H, 4 * L.n_H, "hipMalloc(H)");
-	if (!e) e |= dalloc((void **)&j->part, 4 * L.n_part, "hipMalloc(partials)");
-	if (!e) e |= dalloc((void **)&j->colW, 4 * L.n_colW, "hipMalloc(colW)");
-	if (!e) e |= dalloc((void **)&j->colH, 4 * L.n_colH, "hipMalloc(colH)");
-	if (!e) e |= dalloc((void **)&j->denW, 4 * L.n_den, "hipMalloc(den)");
+	if (!e) e |= dalloc_job(j, (void **)&j->W, 4 * L.n_W, "hipMalloc(W)");
+	if (!e) e |= dalloc_job(j, (void **)&j->H, 4 * L.n_H, "hipMalloc(H)");
+	if (!e) e |= dalloc_job(j, (void **)&j->part, 4 * L.n_part, "hipMalloc(partials)");
+	if (!e) e |= dalloc_job(j, (void **)&j->colW, 4 * L.n_colW, "hipMalloc(colW)");
+	if (!e) e |= dalloc_job(j, (void **)&j->colH, 4 * L.n_colH, "hipMalloc(colH)");
+	if (!e) e |= dalloc_job(j, (void **)&j->denW, 4 * L.n_den, "hipMalloc(den)");
 	if (!e) j->denH = j->denW + L.kt;
-	if (!e) e |= dalloc((void **)&j->dot, 8 * L.n_dot, "hipMalloc(dot)");
-	if (!e) e |= dalloc((void **)&j->cls, 4 * (size_t)m, "hipMalloc(cls)");
-	if (!e) e |= dalloc((void **)&j->last, 4 * (size_t)m, "hipMalloc(last)");
-	if (!e) e |= dalloc((void **)&j->st, sizeof(nmfk_state), "hipMalloc(state)");
+	if (!e) e |= dalloc_job(j, (void **)&j->dot, 8 * L.n_dot, "hipMalloc(dot)");
+	if (!e) e |= dalloc_job(j, (void **)&j->cls, 4 * (size_t)m, "hipMalloc(cls)");
+	if (!e) e |= dalloc_job(j, (void **)&j->last, 4 * (size_t)m, "hipMalloc(last)");
+	if (!e) e |= dalloc_job(j, (void **)&j->st, sizeof(nmfk_state), "hipMalloc(state)");
 	j->ldtW = L.ldtW, j->ldtH = L.ldtH;
 	if (!e && L.n_shadow) {
-		e |= dalloc((void **)&j->Wb, 2 * L.n_shadow, "hipMalloc(BF16 copies)");
+		e |= dalloc_job(j, (void **)&j->Wb, 2 * L.n_shadow, "hipMalloc(BF16 copies)");
 		if (!e) {
 			j->Wtb = j->Wb + (size_t)n * L.kt;
 			j->Hb = j->Wtb + (size_t)L.ldtW * L.kt;
 			j->Htb = j->Hb + (size_t)m * L.kt;
 		}
 	}
-	if (!e) e |= nmfk_hip_fail(hipHostMalloc((void **)&j->h_st, 2 * sizeof(nmfk_state), hipHostMallocDefault),
-				   "hipHostMalloc(state)");
-	if (!e) e |= nmfk_hip_fail(hipStreamCreateWithFlags(&j->stream, hipStreamNonBlocking), "hipStreamCreate");
-	if (!e) e |= nmfk_hip_fail(hipEventCreateWithFlags(&j->ev[0], hipEventDisableTiming), "hipEventCreate");
-	if (!e) e |= nmfk_hip_fail(hipEventCreateWithFlags(&j->ev[1], hipEventDisableTiming), "hipEventCreate");
+	if (stream) {
+		j->stream = stream;   /* borrowed: the creator keeps it */
+		j->borrowed_stream = 1;
+	} else {
+		if (!e) e |= run_resources(j);
+		if (!e) e |= nmfk_hip_fail(hipStreamCreateWithFlags(&j->stream, hipStreamNonBlocking), "hipStreamCreate");
+	}
 	if (e) {
 		char msg[512];
 		snprintf(msg, sizeof msg, "%s", g_err);
@@ -344,19 +415,20 @@ void nmfk_job_destroy(nmfk_job *j)
 		hipGraphExecDestroy(j->chunk_exec);
 	if (j->streamed)
 		nmfk_stream_free(j);
-	if (j->owns_V)
-		hipFree(j->V), hipFree(j->Vt);
-	hipFree(j->W), hipFree(j->H), hipFree(j->part);
-	hipFree(j->colW), hipFree(j->colH), hipFree(j->denW), hipFree(j->dot), hipFree(j->cls), hipFree(j->last);
-	hipFree(j->Wb);
-	hipFree(j->st);
+	if (j->owns_V) {
+		dfree_job(j, j->V);
+		dfree_job(j, j->Vt);
+	}
+	void *bufs[] = {j->W, j->H, j->part, j->colW, j->colH, j->denW, j->dot, j->cls, j->last, j->Wb, j->st};
+	for (size_t i = 0; i < sizeof bufs / sizeof bufs[0]; i++)
+		dfree_job(j, bufs[i]);
 	if (j->h_st)
 		hipHostFree(j->h_st);
 	if (j->ev[0])
 		hipEventDestroy(j->ev[0]);
 	if (j->ev[1])
 		hipEventDestroy(j->ev[1]);
-	if (j->stream)
+	if (j->stream && !j->borrowed_stream)
 		hipStreamDestroy(j->stream);
 	free(j);
 }
@@ -478,16 +550,17 @@ int nmfk_job_gen_V_planted(nmfk_job *j, uint64_t seed, uint32_t kstar, double si
 	if (!v_writable(j))
 		return NMFK_EINVAL;
 	double *scratch = NULL;
-	int e = dalloc((void **)&scratch, 8 * (size_t)kstar * ((size_t)j->n + j->m), "hipMalloc(planted)");
+	int e = dalloc_job(j, (void **)&scratch, 8 * (size_t)kstar * ((size_t)j->n + j->m), "hipMalloc(planted)");
 	if (e)
 		return e;
 	e = nmfk_launch_gen_planted(j->V, j->n, j->m, (uint32_t)j->ldv, kstar, seed, sigma, scratch, j->stream);
 	if (e) {
-		hipFree(scratch);
+		hipStreamSynchronize(j->stream);
+		dfree_job(j, scratch);
 		return nmfk_hip_fail(e, "gen planted");
 	}
 	e = finish_V(j);
-	hipFree(scratch);
+	dfree_job(j, scratch);
 	return e;
 }
 
@@ -712,6 +785,8 @@ int nmfk_job_run(nmfk_job *j, const nmfk_options *opt, nmfk_result *res)
 	}
 	if (j->streamed)
 		return nmfk_stream_run(j, opt, res);
+	if (run_resources(j))
+		return NMFK_EHIP;
 	const uint32_t T = opt->niter_test_conv;
 	const uint32_t niter_conv = opt->nIters / T, niter_rem = opt->nIters % T;
 	if (nmfk_job_refresh_Vt(j))

@@ -16,6 +16,17 @@
 #define NMFK_ENOMEM 4
 #define NMFK_EBUDGET 5
 
+/* Device memory arena (nmfk_arena.c): segments reserved once, sub-allocated per job. */
+typedef struct nmfk_arena nmfk_arena;
+nmfk_arena *nmfk_arena_create(size_t limit, size_t seg_bytes);
+void nmfk_arena_destroy(nmfk_arena *a);
+void *nmfk_arena_alloc(nmfk_arena *a, size_t bytes);   /* NULL when it does not fit */
+int nmfk_arena_free(nmfk_arena *a, void *p);
+size_t nmfk_arena_round(size_t bytes);                  /* bytes a block of this size takes */
+size_t nmfk_arena_reserved(const nmfk_arena *a);
+size_t nmfk_arena_in_use(const nmfk_arena *a);
+void nmfk_arena_trim(nmfk_arena *a);                   /* releases segments with nothing in use */
+
 struct nmfk_job {
 	uint32_t n, m, k, kt;
 	size_t ldv, ldvt;
@@ -63,6 +74,8 @@ struct nmfk_job {
 	int slot0_written;             /* a caller wrote slot 0 on the job's stream (drop-in upload) */
 	/* Per-test connectivity differences of the current run (nmfk_job_run_traced): the host
 	 * copies the device state after every test interval and keeps its last_diff. */
+	nmfk_arena *arena;             /* buffers come from (and return to) this arena; NULL: hipMalloc */
+	int borrowed_stream;           /* the stream belongs to the job's creator */
 	int tracing;
 	uint64_t *diff_trace;
 	uint32_t trace_cap, trace_len;
@@ -77,6 +90,14 @@ static inline void nmfk_trace_state(nmfk_job *j, const nmfk_state *s)
 		j->trace_len++;
 	}
 }
+
+/* A job whose device buffers come from <arena> and whose work runs on the caller's <stream>
+ * (nmfk_job_create, or nmfk_job_create_shared when src is given, without any hipMalloc,
+ * hipFree or stream creation; nmfk_job_destroy returns the buffers to the arena). */
+nmfk_job *nmfk_job_create_in(nmfk_arena *arena, hipStream_t stream, uint32_t n, uint32_t m, uint32_t k,
+			     nmfk_job *src);
+/* nmfk_job_bytes without V and Vt: a job sharing another job's V. */
+size_t nmfk_job_bytes_shared(uint32_t n, uint32_t m, uint32_t k);
 
 /* Streamed jobs: the loop and the Frobenius/KL pass (nmfk_stream.c). */
 int nmfk_stream_run(nmfk_job *j, const nmfk_options *opt, nmfk_result *res);

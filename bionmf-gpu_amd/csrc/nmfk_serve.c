@@ -13,6 +13,11 @@
  *
  * The host loop enqueues the next chunk before doing admission work (matrix generation,
  * job set-up), so the device keeps iterating while new requests are prepared.
+ *
+ * Device memory comes from an arena reserved within the HBM budget (nmfk_arena.c), and all
+ * set-up work runs on one stream created before the loop: admitting and retiring requests
+ * calls no hipMalloc, hipFree or stream creation, so the host thread never waits for the
+ * device to drain (hipFree would).
  */
 #include "nmfk_internal.h"
 
@@ -37,13 +42,10 @@ void nmfk_serve_default_config(nmfk_serve_config *c)
 }
 
 /* Device bytes of one request: its own job (V, Vt and buffers) plus restarts - 1 jobs that
- * share V. */
+ * share V, each buffer counted as the arena places it. */
 size_t nmfk_request_bytes(uint32_t n, uint32_t m, uint32_t k, uint32_t restarts)
 {
-	const size_t full = nmfk_job_bytes(n, m, k);
-	const size_t ldv = (m + 3u) & ~3u, ldvt = (n + 3u) & ~3u;
-	const size_t vbytes = 4 * ((size_t)n * ldv + (size_t)m * ldvt);
-	return full + (size_t)(restarts ? restarts - 1 : 0) * (full - vbytes);
+	return nmfk_job_bytes(n, m, k) + (size_t)(restarts ? restarts - 1 : 0) * nmfk_job_bytes_shared(n, m, k);
 }
 
 typedef struct req_state {
@@ -92,8 +94,10 @@ static void sleep_until(double t0, double t)
 	nanosleep(&ts, NULL);
 }
 
-/* Creates the request's jobs: V from its seed, restart r from nmfk_sweep_seed(seed, k, r). */
-static int admit(const nmfk_serve_config *cfg, const nmfk_request *q, req_state *r, nmfk_batch *b)
+/* Creates the request's jobs: V from its seed, restart r from nmfk_sweep_seed(seed, k, r).
+ * Their buffers come from the arena and their set-up runs on the serving loop's stream. */
+static int admit(const nmfk_serve_config *cfg, const nmfk_request *q, req_state *r, nmfk_batch *b,
+		 nmfk_arena *arena, hipStream_t stream)
 {
 	r->jobs = (nmfk_job **)calloc(q->restarts, sizeof(nmfk_job *));
 	if (!r->jobs) {
@@ -106,15 +110,17 @@ static int admit(const nmfk_serve_config *cfg, const nmfk_request *q, req_state 
 	opt.niter_test_conv = cfg->niter_test_conv;
 	opt.stop_threshold = cfg->stop_threshold;
 	int e = 0;
-	r->jobs[0] = nmfk_job_create(q->n, q->m, q->k);
+	r->jobs[0] = nmfk_job_create_in(arena, stream, q->n, q->m, q->k, NULL);
 	if (!r->jobs[0])
-		return NMFK_EINVAL;
+		return NMFK_EBUDGET;
 	e = nmfk_job_gen_V_lognormal(r->jobs[0], q->seed, cfg->mu, cfg->sigma);
 	for (uint32_t i = 1; i < q->restarts && !e; i++) {
-		r->jobs[i] = nmfk_job_create_shared(r->jobs[0], q->k);
+		r->jobs[i] = nmfk_job_create_in(arena, stream, q->n, q->m, q->k, r->jobs[0]);
 		if (!r->jobs[i])
-			e = NMFK_EINVAL;
+			e = NMFK_EBUDGET;
 	}
+	if (e)
+		return e;   /* nothing was added to the batch yet */
 	for (uint32_t i = 0; i < q->restarts && !e; i++) {
 		if (cfg->plan_target)
 			e = nmfk_job_replan(r->jobs[i], cfg->plan_target);
@@ -126,8 +132,33 @@ static int admit(const nmfk_serve_config *cfg, const nmfk_request *q, req_state 
 	return e;
 }
 
+/* Copies a retired restart's outcome into the caller's record (buffers optional). */
+static int record_restart(nmfk_job *j, const nmfk_result *r, double kl, nmfk_restart_result *o)
+{
+	o->iterations = r->iterations;
+	o->converged = r->converged;
+	o->ntests = r->ntests;
+	o->last_diff = r->last_diff;
+	o->kl = kl;
+	int e = 0;
+	/* the class vector of the final H, as nmfk_job_get_classification gives it */
+	if (o->cls && (e = nmfk_job_get_classification(j, o->cls)) != 0)
+		return e;
+	if (o->W && (e = nmfk_job_get_W(j, o->W, j->k)) != 0)
+		return e;
+	if (o->H && (e = nmfk_job_get_H(j, o->H, j->k)) != 0)
+		return e;
+	return nmfk_job_sync(j);
+}
+
 int nmfk_serve_run(const nmfk_serve_config *cfg, const nmfk_request *req, uint32_t nreq, nmfk_request_result *out,
 		   nmfk_serve_stats *stats)
+{
+	return nmfk_serve_run_detail(cfg, req, nreq, out, stats, NULL);
+}
+
+int nmfk_serve_run_detail(const nmfk_serve_config *cfg, const nmfk_request *req, uint32_t nreq,
+			  nmfk_request_result *out, nmfk_serve_stats *stats, nmfk_restart_result *restarts)
 {
 	for (uint32_t i = 0; i < nreq; i++) {
 		const nmfk_request *q = &req[i];
@@ -144,16 +175,41 @@ int nmfk_serve_run(const nmfk_serve_config *cfg, const nmfk_request *req, uint32
 	nmfk_batch *b = nmfk_batch_create(cfg->niter_test_conv, cfg->capacity);
 	if (!b)
 		return NMFK_EINVAL;
+	/* The arena's first segment is reserved here, before the clock starts: the largest
+	 * request, at least 16 GiB, within the budget. */
+	size_t first_seg = (size_t)16 << 30;
+	for (uint32_t i = 0; i < nreq; i++) {
+		const size_t need = nmfk_request_bytes(req[i].n, req[i].m, req[i].k, req[i].restarts);
+		if (need > first_seg)
+			first_seg = need;
+	}
+	if (first_seg > cfg->hbm_budget)
+		first_seg = cfg->hbm_budget;
+	nmfk_arena *arena = nmfk_arena_create(cfg->hbm_budget, first_seg);
+	hipStream_t stream = NULL;
 	req_state *rs = (req_state *)calloc(nreq ? nreq : 1, sizeof(req_state));
 	nmfk_job **done = (nmfk_job **)malloc(cfg->capacity * sizeof(nmfk_job *));
 	nmfk_result *dres = (nmfk_result *)malloc(cfg->capacity * sizeof(nmfk_result));
 	double *lat = (double *)malloc((nreq ? nreq : 1) * sizeof(double));
+	size_t *roff = (size_t *)malloc((nreq ? nreq : 1) * sizeof(size_t));   /* first restart record of request i */
 	/* job -> request map, by slot-independent linear search over active requests' jobs */
-	if (!rs || !done || !dres || !lat) {
-		free(rs), free(done), free(dres), free(lat);
+	if (!arena || !rs || !done || !dres || !lat || !roff) {
+		free(rs), free(done), free(dres), free(lat), free(roff);
+		nmfk_arena_destroy(arena);
 		nmfk_batch_destroy(b);
 		nmfk_set_error("out of host memory");
 		return NMFK_ENOMEM;
+	}
+	for (uint32_t i = 0, o = 0; i < nreq; o += req[i].restarts, i++)
+		roff[i] = o;
+	int e = nmfk_hip_fail(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate(serve)");
+	if (!e && nreq) {   /* reserve the first segment now */
+		void *p = nmfk_arena_alloc(arena, first_seg);
+		if (!p) {
+			nmfk_set_error("nmfk_serve_run: cannot reserve %zu bytes of device memory", first_seg);
+			e = NMFK_EBUDGET;
+		}
+		nmfk_arena_free(arena, p);
 	}
 	for (uint32_t i = 0; i < nreq; i++) {
 		memset(&out[i], 0, sizeof out[i]);
@@ -161,7 +217,6 @@ int nmfk_serve_run(const nmfk_serve_config *cfg, const nmfk_request *req, uint32
 	}
 	size_t used = 0, peak = 0;
 	uint32_t active_jobs = 0, peak_jobs = 0, next = 0, finished = 0, first_active = 0;
-	int e = 0;
 	const double t0 = nmfk_now();
 	while (!e && finished < nreq) {
 		/* 1. keep the device busy */
@@ -176,7 +231,18 @@ int nmfk_serve_run(const nmfk_serve_config *cfg, const nmfk_request *req, uint32
 			const size_t need = nmfk_request_bytes(q->n, q->m, q->k, q->restarts);
 			if (used + need > cfg->hbm_budget || active_jobs + q->restarts > cfg->capacity)
 				break;
-			e = admit(cfg, q, &rs[next], b);
+			e = admit(cfg, q, &rs[next], b, arena, stream);
+			if (e == NMFK_EBUDGET && rs[next].jobs && rs[next].jobs[q->restarts - 1] == NULL) {
+				/* The budget has room but the arena has no block that large (its free space is
+				 * split): undo, and wait for retirements; with nothing running, release the
+				 * arena's empty segments and try once more. */
+				free_request(&rs[next], q->restarts);
+				e = 0;
+				if (active_jobs > 0)
+					break;
+				nmfk_arena_trim(arena);
+				e = admit(cfg, q, &rs[next], b, arena, stream);
+			}
 			if (e)
 				break;
 			rs[next].bytes = need;
@@ -223,6 +289,8 @@ int nmfk_serve_run(const nmfk_serve_config *cfg, const nmfk_request *req, uint32
 			nmfk_request_result *o = &out[qi];
 			double kl = 0.0;
 			e = nmfk_job_dot_kl(done[d], NULL, NULL, &kl);
+			if (!e && restarts)
+				e = record_restart(done[d], &dres[d], kl, &restarts[roff[qi] + ri]);
 			if (e)
 				break;
 			if (dres[d].iterations > o->iterations_max)
@@ -255,6 +323,9 @@ int nmfk_serve_run(const nmfk_serve_config *cfg, const nmfk_request *req, uint32
 	nmfk_batch_destroy(b);
 	for (uint32_t i = 0; i < nreq; i++)
 		free_request(&rs[i], req[i].restarts);
+	if (stream)
+		hipStreamDestroy(stream);
+	nmfk_arena_destroy(arena);
 	if (!e && stats) {
 		qsort(lat, finished, sizeof(double), cmp_double);
 		uint64_t jobs = 0;
@@ -274,6 +345,6 @@ int nmfk_serve_run(const nmfk_serve_config *cfg, const nmfk_request *req, uint32
 		stats->peak_bytes = peak;
 		stats->peak_jobs = peak_jobs;
 	}
-	free(rs), free(done), free(dres), free(lat);
+	free(rs), free(done), free(dres), free(lat), free(roff);
 	return e;
 }

@@ -283,6 +283,21 @@ size_t nmfk_request_bytes(uint32_t n, uint32_t m, uint32_t k, uint32_t restarts)
 int nmfk_serve_run(const nmfk_serve_config *cfg, const nmfk_request *req, uint32_t nreq,
 		   nmfk_request_result *out, nmfk_serve_stats *stats);
 
+/* Outcome of one restart of a served request (nmfk_serve_run_detail). */
+typedef struct nmfk_restart_result {
+	uint32_t iterations, converged, ntests, pad_;   /* as nmfk_result */
+	uint64_t last_diff;
+	double kl;                    /* D(V || WH) of the final factors, FP64 */
+	uint32_t *cls;                /* caller's buffer of m entries for the final class vector, or NULL */
+	float *W, *H;                 /* caller's buffers (n x k, m x k) for the final factors, or NULL */
+} nmfk_restart_result;
+
+/* nmfk_serve_run that also fills one record per restart: request i's restart r is
+ * restarts[sum of the restarts of requests 0..i-1 + r].  The cls, W and H pointers of each
+ * record are set by the caller (NULL: not copied). */
+int nmfk_serve_run_detail(const nmfk_serve_config *cfg, const nmfk_request *req, uint32_t nreq,
+			  nmfk_request_result *out, nmfk_serve_stats *stats, nmfk_restart_result *restarts);
+
 /* Raw device pointers and shape (for the drop-in layer and for measurements). */
 typedef struct nmfk_job_info {
 	uint32_t n, m, k, kt;

@@ -1,8 +1,10 @@
 """Writes the golden fixtures under tests/golden/ from the CPU oracle (and converts the
 reference's ALL_AML test matrix into a compact array).
 
-    python tests/golden/make_golden.py            # rewrite fixtures
+    python tests/golden/make_golden.py            # rewrite the small fixtures
     python tests/golden/make_golden.py --ref DIR  # reference tree (default /root/reference)
+    python tests/golden/make_golden.py --large    # also the two FP64 runs at full size
+                                                  # (about 2 and 12 minutes on 8 cores)
 
 Fixtures:
   tiny.npz       small shapes: V, W0, H0 (Philox seeds), W/H after 1 and 10 iterations of
@@ -11,6 +13,12 @@ Fixtures:
   config0.npz    5000 x 38 lognormal(5, 1.5^2) V, k=3: class vector, stop iteration and KL
                  of the FP64 and FP32 oracle runs (T=10, stop_threshold=40, 2000 iterations).
   all_aml.npz    test/ALL_AML_data_noLabels.txt of the reference (5000 x 38 integers) as uint16.
+  config1_fp64.npz  the single large job at full size: V 20000 x 1000 planted rank 16
+                 (seed 20250101, sigma 0.3), k = 16, W0/H0 from seed 7, 500 iterations with a
+                 test every 10 and no stop: FP64 oracle's KL, Frobenius sums, class vector,
+                 per-test differences, H and every 20th row of W.
+  config3_fp64.npz  the dense-rank job: V 50000 x 2000 planted rank 48 (seed 4848), k = 64,
+                 W0/H0 from seed 9, 300 iterations: the same quantities, every 25th row of W.
 """
 from __future__ import annotations
 
@@ -78,6 +86,31 @@ def config0():
     return out
 
 
+LARGE = {  # name: n, m, k, kstar, seed_v, seed_init, iterations, W row step
+    "config1_fp64": (20000, 1000, 16, 16, 20250101, 7, 500, 20),
+    "config3_fp64": (50000, 2000, 64, 48, 4848, 9, 300, 25),
+}
+
+
+def large(name):
+    """FP64 oracle run of a full-size workload (V planted, sigma 0.3, test every 10
+    iterations, never stopping early), reduced to what a device run is compared with."""
+    n, m, k, kstar, sv, si, iters, step = LARGE[name]
+    V = O.gen_planted(n, m, kstar, sv, 0.3)
+    W0, H0 = O.init_WH(n, m, k, si)
+    r = O.nmf(V, W0, H0, iters, 10, 10**9, fp64=True, max_trace=iters // 10 + 1)
+    dv, dvwh = O.dot_VWH(V, r["W"], r["H"])
+    rows = np.arange(0, n, step)
+    probe = [(i * 7919) % n for i in range(64)]
+    return {"params": np.array([n, m, k, kstar, sv, si, iters, step]),
+            "V_checksum": np.array(float(V.astype(np.float64).sum())),
+            "V_probe_idx": np.array(probe), "V_probe": V[probe, [(i * 104729) % m for i in range(64)]],
+            "kl": np.array(O.kl_div(V, r["W"], r["H"], fp64=True)), "dot_V": np.array(dv),
+            "dot_VWH": np.array(dvwh), "cls": r["cls"], "ntests": np.array(r["ntests"]),
+            "iterations": np.array(r["iterations"]), "diff_trace": r["diff_trace"],
+            "H": r["H"].astype(np.float32), "W_rows": r["W"][rows].astype(np.float32)}
+
+
 def all_aml(ref: pathlib.Path):
     src = ref / "test" / "ALL_AML_data_noLabels.txt"
     A = np.loadtxt(src, dtype=np.float64)
@@ -88,7 +121,15 @@ def all_aml(ref: pathlib.Path):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--ref", default="/root/reference")
+    ap.add_argument("--large", action="store_true", help="also the full-size FP64 runs")
+    ap.add_argument("--only-large", action="store_true", help="only the full-size FP64 runs")
     args = ap.parse_args()
+    if args.large or args.only_large:
+        for name in LARGE:
+            np.savez_compressed(HERE / f"{name}.npz", **large(name))
+            print("written", name, flush=True)
+    if args.only_large:
+        return
     np.savez_compressed(HERE / "tiny.npz", **tiny())
     np.savez_compressed(HERE / "config0.npz", **config0())
     ref = pathlib.Path(args.ref)

@@ -305,3 +305,71 @@ def test_transpose_beyond_65535_row_tiles(nmfk, orc):
         W = j.get_W()
     W0, H0 = orc.init_WH(n, m, k, 5)
     assert rel_err(W, orc.update_W(V, W0, H0)) < 1e-5
+
+
+# ---------------------------------------------------------------- full-size runs against FP64
+
+def _large_fixture(name):
+    f = GOLDEN / f"{name}.npz"
+    if not f.exists():
+        pytest.skip(f"{f.name} not generated (tests/golden/make_golden.py --large)")
+    return np.load(f)
+
+
+def _device_run_vs_fp64(nmfk, g, path=None, trace=True):
+    n, m, k, kstar, sv, si, iters, step = (int(x) for x in g["params"])
+    with nmfk.Job(n, m, k) as job:
+        job.gen_V_planted(sv, kstar, 0.3)
+        V = job.get_V()
+        # the device generated the oracle's V
+        assert float(V.astype(np.float64).sum()) == float(g["V_checksum"])
+        assert np.array_equal(V[g["V_probe_idx"], [(i * 104729) % m for i in range(64)]], g["V_probe"])
+        del V
+        if path is not None:
+            job.set_kernel_path(path)
+        job.init_random(si)
+        r = job.run(iters, 10, 10**9, trace=trace)
+        dv, dvwh, kl = job.dot_kl()
+        W, H, cls = job.get_W(), job.get_H(), job.get_classification()
+    rows = np.arange(0, n, step)
+    WH = W[rows].astype(np.float64) @ H.T.astype(np.float64)
+    WHr = g["W_rows"].astype(np.float64) @ g["H"].T.astype(np.float64)
+    err = {"rel_kl": abs(kl - float(g["kl"])) / float(g["kl"]),
+           "rel_frobenius_WH_sample": float(np.linalg.norm(WH - WHr) / np.linalg.norm(WHr)),
+           "rel_H": rel_err(H, g["H"]), "rel_W_sample": rel_err(W[rows], g["W_rows"]),
+           "rel_dist": abs(np.sqrt(dvwh / dv) - np.sqrt(float(g["dot_VWH"]) / float(g["dot_V"])))
+           / np.sqrt(float(g["dot_VWH"]) / float(g["dot_V"])),
+           "class_agreement": float(np.mean(cls == g["cls"]))}
+    return r, cls, err
+
+
+def test_config1_full_run_against_fp64(nmfk):
+    # The benchmarked workload itself: V 20000 x 1000 planted rank 16, k = 16, 500 iterations
+    # (FP32 matrix cores), against the FP64 oracle run from the same W0, H0.
+    g = _large_fixture("config1_fp64")
+    r, cls, err = _device_run_vs_fp64(nmfk, g)
+    print("config1 FP32 vs FP64:", err)
+    assert r["iterations"] == int(g["iterations"]) and r["ntests"] == int(g["ntests"])
+    assert err["rel_kl"] <= 1e-3 and err["rel_dist"] <= 1e-3
+    assert err["rel_frobenius_WH_sample"] <= 1e-3
+    assert np.array_equal(cls, g["cls"])
+    assert np.array_equal(r["diff_trace"], g["diff_trace"])
+
+
+@pytest.mark.parametrize("path", [3, 2], ids=["bf16", "fp32"])
+def test_config3_full_run_against_fp64(nmfk, path):
+    # Dense rank: V 50000 x 2000 planted rank 48, k = 64, 300 iterations.  BF16 contractions
+    # with FP32 accumulation (path 3, the default at k = 64) and FP32 matrix cores (path 2)
+    # against FP64.  Stated bounds: KL within 1e-3 relative on both paths; W.H (every 25th
+    # row) within 1e-3 relative Frobenius; class vectors: FP32 equal to FP64, BF16 at least
+    # 99% equal (its products round the operands to 8 significant bits).
+    g = _large_fixture("config3_fp64")
+    r, cls, err = _device_run_vs_fp64(nmfk, g, path=path, trace=False)
+    print(f"config3 path {path} vs FP64:", err)
+    assert r["iterations"] == int(g["iterations"])
+    assert err["rel_kl"] <= 1e-3 and err["rel_dist"] <= 1e-3
+    assert err["rel_frobenius_WH_sample"] <= 1e-3
+    if path == 2:
+        assert np.array_equal(cls, g["cls"])
+    else:
+        assert err["class_agreement"] >= 0.99

@@ -178,6 +178,8 @@ def load(path: os.PathLike | str | None = None):
     _sig(lib, "nmfk_job_get_classification", i32, [vp, _U32P])
     _sig(lib, "nmfk_job_dot_kl", i32, [vp, C.POINTER(dbl), C.POINTER(dbl), C.POINTER(dbl)])
     _sig(lib, "nmfk_job_sync", i32, [vp])
+    _sig(lib, "nmfk_dot_kl_noalloc", i32, [vp, sz, vp, vp, u32, u32, u32, vp, vp])
+    _sig(lib, "nmfk_dot_kl_noalloc_max_bytes", sz, [])
     _sig(lib, "nmfk_job_info_get", i32, [vp, C.POINTER(JobInfo)])
     _sig(lib, "nmfk_job_time_iterations", i32, [vp, u32, C.POINTER(Timing)])
     _sig(lib, "nmfk_job_set_kernel_path", i32, [vp, i32])

@@ -101,6 +101,20 @@ int nmfk_job_get_classification(nmfk_job *job, uint32_t *cls);
 /* FP64 sums: <V,V>, <V-WH,V-WH> and the generalized KL divergence D(V||WH). */
 int nmfk_job_dot_kl(nmfk_job *job, double *dot_V, double *dot_VWH, double *kl);
 
+/* nmfk_job_dot_kl on caller-owned device buffers, allocating nothing.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  V is n x ldv
+ * (ldv >= m), W n x kt and H m x kt with kt = nmfk_kt(k) and columns k..kt-1 zero.  Writes
+ * {<V,V>, <V-WH,V-WH>, D(V||WH)} as 3 doubles to device memory <out>, asynchronously on
+ * <stream> (a hipStream_t; NULL = the null stream): the same bits nmfk_job_dot_kl returns
+ * for the same buffers.  The FP64 partial sums go to a static device array of
+ * nmfk_dot_kl_noalloc_max_bytes() bytes (1 MiB) per device, held until the process ends;
+ * 24 bytes per 256 columns x 64 row splits, so m <= 174592; larger m is refused.  Calls
+ * that overlap on one device share that array and race: serialise them. */
+int nmfk_dot_kl_noalloc(const float *dV, size_t ldv, const float *dW, const float *dH, uint32_t n, uint32_t m,
+			uint32_t k, double *d_out, void *stream);
+size_t nmfk_dot_kl_noalloc_max_bytes(void);
+
 /* Synchronizes the job's stream. */
 int nmfk_job_sync(nmfk_job *job);
 

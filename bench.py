@@ -449,6 +449,24 @@ def main():
     if tfile.exists() and args.kernel_path == "auto":
         traffic = json.loads(tfile.read_text()).get("bytes_per_launch")
 
+    # The one-pass schedule (one pass over V per iteration: k_fused_mfma + k_fused_finish) on
+    # the same job, timed the same way; reported beside the default two-pass schedule.
+    one_pass = None
+    if args.kernel_path in ("auto", "mfma32"):
+        job.set_schedule(job.SCHED_ONE_PASS)
+        job.init_random(SEED_INIT)
+        job.time_one_pass(5)
+        t1 = job.time_one_pass(50)
+        job.init_random(SEED_INIT)
+        r1 = job.run(ITERS, T, 1_000_000_000)
+        pass_bytes = 4.0 * (N * M + 2 * N * K + 2 * M * K)
+        one_pass = {"ms_per_iteration": t1["ms_per_iter"], "ms_pass": t1["ms_pass"], "ms_finish": t1["ms_finish"],
+                    "workgroups": t1["workgroups"], "waves_per_workgroup": t1["waves_per_workgroup"],
+                    "pass_bytes": pass_bytes, "pass_GBps": pass_bytes / (t1["ms_pass"] * 1e-3) / 1e9,
+                    "pass_matrix_TFLOPs": 8.0 * N * M * K / (t1["ms_pass"] * 1e-3) / 1e12,
+                    "factorization_s": r1["seconds"]}
+        job.set_schedule(job.SCHED_AUTO)
+
     extra = {}
     if not args.no_extra:
         extra["c0"] = bench_c0(pkg)
@@ -500,6 +518,7 @@ def main():
                      "measured_read_GBps": read_gbs, "measured_copy_GBps": copy_gbs,
                      "frac_of_measured_read": achieved / read_gbs},
         "cpu_baseline": cpu,
+        "one_pass_schedule": one_pass,
     }
     line.update(extra)
     job.close()

@@ -57,6 +57,11 @@ class Timing(C.Structure):
                 ("ms_finish_W", C.c_double)]
 
 
+class TimingOnePass(C.Structure):
+    _fields_ = [("ms_per_iter", C.c_double), ("ms_pass", C.c_double), ("ms_finish", C.c_double),
+                ("workgroups", C.c_uint32), ("waves_per_workgroup", C.c_uint32)]
+
+
 class JobInfo(C.Structure):
     _fields_ = [("n", C.c_uint32), ("m", C.c_uint32), ("k", C.c_uint32), ("kt", C.c_uint32),
                 ("ldv", C.c_size_t), ("ldvt", C.c_size_t),
@@ -183,6 +188,9 @@ def load(path: os.PathLike | str | None = None):
     _sig(lib, "nmfk_job_info_get", i32, [vp, C.POINTER(JobInfo)])
     _sig(lib, "nmfk_job_time_iterations", i32, [vp, u32, C.POINTER(Timing)])
     _sig(lib, "nmfk_job_set_kernel_path", i32, [vp, i32])
+    _sig(lib, "nmfk_job_set_schedule", i32, [vp, i32])
+    _sig(lib, "nmfk_job_schedule", i32, [vp])
+    _sig(lib, "nmfk_job_time_one_pass", i32, [vp, u32, C.POINTER(TimingOnePass)])
     _sig(lib, "nmfk_job_kernel_path", i32, [vp])
     _sig(lib, "nmfk_tune_set", i32, [i32, i32])
     _sig(lib, "nmfk_tune_get", i32, [i32])
@@ -474,6 +482,19 @@ class Job:
 
     def kernel_path(self) -> int:
         return self.lib.nmfk_job_kernel_path(self.h)
+
+    SCHED_AUTO, SCHED_TWO_PASS, SCHED_ONE_PASS = 0, 1, 2
+
+    def set_schedule(self, sched: int):
+        _check(self.lib.nmfk_job_set_schedule(self.h, sched), "set_schedule")
+
+    def schedule(self) -> int:
+        return self.lib.nmfk_job_schedule(self.h)
+
+    def time_one_pass(self, iters: int) -> dict:
+        t = TimingOnePass()
+        _check(self.lib.nmfk_job_time_one_pass(self.h, iters, C.byref(t)), "time_one_pass")
+        return {f: getattr(t, f) for f, _ in TimingOnePass._fields_}
 
     def info(self) -> JobInfo:
         inf = JobInfo()

@@ -419,7 +419,8 @@ void nmfk_job_destroy(nmfk_job *j)
 		dfree_job(j, j->V);
 		dfree_job(j, j->Vt);
 	}
-	void *bufs[] = {j->W, j->H, j->part, j->colW, j->colH, j->denW, j->dot, j->cls, j->last, j->Wb, j->st};
+	void *bufs[] = {j->W, j->H, j->part, j->colW, j->colH, j->denW, j->dot, j->cls, j->last, j->Wb, j->st,
+			j->fpart, j->fcolW};
 	for (size_t i = 0; i < sizeof bufs / sizeof bufs[0]; i++)
 		dfree_job(j, bufs[i]);
 	if (j->h_st)
@@ -485,6 +486,88 @@ int nmfk_job_replan(nmfk_job *j, uint32_t target_wgs)
 		j->chunk_exec = NULL;
 	}
 	return 0;
+}
+
+/* ---------------------------------------------------------------- iteration schedule */
+
+/* The one-pass kernel runs the job: V on the device, both half-steps planned on the FP32
+ * matrix cores, and a shape k_fused_mfma takes. */
+static int one_pass_ok(const nmfk_job *j, uint32_t *G, uint32_t *nwv)
+{
+	if (j->streamed || !j->V || j->planH.path != NMFK_PATH_MFMA32 || j->planW.path != NMFK_PATH_MFMA32)
+		return 0;
+	return nmfk_fused_plan(j->n, j->m, j->kt, G, nwv) == 0;
+}
+
+int nmfk_job_set_schedule(nmfk_job *j, int sched)
+{
+	uint32_t G, nwv;
+	if (sched < NMFK_SCHED_AUTO || sched > NMFK_SCHED_ONE_PASS) {
+		nmfk_set_error("nmfk_job_set_schedule: unknown schedule %d", sched);
+		return NMFK_EINVAL;
+	}
+	if (sched == NMFK_SCHED_ONE_PASS && !one_pass_ok(j, &G, &nwv)) {
+		nmfk_set_error("nmfk_job_set_schedule: one pass per iteration needs V on the device, the FP32 "
+			       "matrix-core path at padded rank 16 (k 9..16) and m <= 1024 (job %ux%u, k=%u)",
+			       j->n, j->m, j->k);
+		return NMFK_EINVAL;
+	}
+	j->sched = sched;
+	return 0;
+}
+
+/* Automatic is two passes: on one MI355X the one-pass kernel measured from 12% faster to
+ * 30% slower than two passes over 20000-80000 x 200-1024 at k = 16 (slower at the benchmarked
+ * 20000 x 1000), since both are bound by the FP32 matrix-core work, which fusing does not
+ * reduce.  A requested one pass falls back to two where it cannot run (kernel path changed). */
+int nmfk_job_schedule(const nmfk_job *j)
+{
+	uint32_t G, nwv;
+	if (j->sched == NMFK_SCHED_ONE_PASS && one_pass_ok(j, &G, &nwv))
+		return NMFK_SCHED_ONE_PASS;
+	return NMFK_SCHED_TWO_PASS;
+}
+
+/* The pass's sums and column sums, sized by the current plan (reallocated, and the captured
+ * interval graph dropped, when a tunable changed the number of workgroups). */
+static int one_pass_buffers(nmfk_job *j)
+{
+	uint32_t G = 0, nwv = 0;
+	if (!one_pass_ok(j, &G, &nwv)) {
+		nmfk_set_error("one pass per iteration is not available for this job");
+		return NMFK_EINVAL;
+	}
+	if (j->fpart && j->fG == G && j->fnwv == nwv)
+		return 0;
+	dfree_job(j, j->fpart);
+	dfree_job(j, j->fcolW);
+	j->fpart = j->fcolW = NULL;
+	j->fG = j->fnwv = 0;
+	if (j->chunk_exec) {
+		hipGraphExecDestroy(j->chunk_exec);
+		j->chunk_exec = NULL;
+	}
+	int e = dalloc_job(j, (void **)&j->fpart, 4 * (size_t)G * j->m * j->kt, "hipMalloc(one-pass sums)");
+	if (!e)
+		e = dalloc_job(j, (void **)&j->fcolW, 4 * (size_t)G * j->kt, "hipMalloc(one-pass column sums)");
+	if (e)
+		return e;
+	j->fG = G;
+	j->fnwv = nwv;
+	return 0;
+}
+
+/* The pass over V: new W rows (upd_w) and the next H half-step's sums; clamp on test passes. */
+static int one_pass(nmfk_job *j, int upd_w, int clamp, const nmfk_state *st)
+{
+	return nmfk_launch_fused(j->V, j->ldv, j->W, j->H, j->n, j->m, j->k, j->kt, j->fG, j->fnwv, j->fpart,
+				 j->fcolW, upd_w, clamp, st, j->stream);
+}
+
+/* The H half-step from the last pass's sums. */
+static int one_pass_finish(nmfk_job *j, const nmfk_state *st)
+{
+	return nmfk_launch_fused_finish(j->fpart, j->fG, j->fcolW, j->H, j->m, j->k, j->kt, st, j->stream);
 }
 
 int nmfk_job_plan_get(const nmfk_job *j, uint32_t out[8])
@@ -738,10 +821,19 @@ int nmfk_job_get_classification(nmfk_job *j, uint32_t *cls)
 
 /* ---------------------------------------------------------------- the loop */
 
-static int enqueue_interval(nmfk_job *j, uint32_t T)
+/* One test interval: T iterations, the last of which clamps (src/NMF_GPU.c:611-638), then
+ * the test.  One pass per iteration: the H update from the previous pass's sums, then the
+ * pass (new W rows, and the sums for the next H update). */
+static int enqueue_interval(nmfk_job *j, uint32_t T, int sched)
 {
 	int e = 0;
 	for (uint32_t i = 0; i < T && !e; i++) {
+		if (sched == NMFK_SCHED_ONE_PASS) {
+			e = one_pass_finish(j, j->st);
+			if (!e)
+				e = one_pass(j, 1, i + 1 == T, j->st);
+			continue;
+		}
 		e = nmfk_enqueue_half_H(j, j->st);
 		if (!e)
 			e = nmfk_enqueue_half_W(j, j->st, i + 1 == T);
@@ -751,9 +843,9 @@ static int enqueue_interval(nmfk_job *j, uint32_t T)
 	return e;
 }
 
-static int build_interval_graph(nmfk_job *j, uint32_t T)
+static int build_interval_graph(nmfk_job *j, uint32_t T, int sched)
 {
-	if (j->chunk_exec && j->chunk_T == T)
+	if (j->chunk_exec && j->chunk_T == T && j->chunk_sched == sched)
 		return 0;
 	if (j->chunk_exec) {
 		hipGraphExecDestroy(j->chunk_exec);
@@ -763,7 +855,7 @@ static int build_interval_graph(nmfk_job *j, uint32_t T)
 	int e = hipStreamBeginCapture(j->stream, hipStreamCaptureModeThreadLocal);
 	if (e)
 		return nmfk_hip_fail(e, "hipStreamBeginCapture");
-	int e2 = enqueue_interval(j, T);
+	int e2 = enqueue_interval(j, T, sched);
 	e = hipStreamEndCapture(j->stream, &g);
 	if (e2)
 		return nmfk_hip_fail(e2, "capture interval");
@@ -774,6 +866,7 @@ static int build_interval_graph(nmfk_job *j, uint32_t T)
 	if (e)
 		return nmfk_hip_fail(e, "hipGraphInstantiate");
 	j->chunk_T = T;
+	j->chunk_sched = sched;
 	return 0;
 }
 
@@ -789,16 +882,27 @@ int nmfk_job_run(nmfk_job *j, const nmfk_options *opt, nmfk_result *res)
 		return NMFK_EHIP;
 	const uint32_t T = opt->niter_test_conv;
 	const uint32_t niter_conv = opt->nIters / T, niter_rem = opt->nIters % T;
-	if (nmfk_job_refresh_Vt(j))
+	const int sched = nmfk_job_schedule(j);
+	if (sched == NMFK_SCHED_ONE_PASS) {
+		if (one_pass_buffers(j))
+			return NMFK_EHIP;
+	} else if (nmfk_job_refresh_Vt(j)) {
 		return NMFK_EHIP;
+	}
 	int e = nmfk_launch_state_reset(j->st, opt->stop_threshold, UINT32_MAX, j->last, j->m, j->stream);
-	if (!e)
-		e = nmfk_launch_colsum_parts(j->W, j->n, j->kt, j->colW, j->stream);
-	if (!e)
-		e = nmfk_job_shadow_for_H(j);
+	if (sched == NMFK_SCHED_ONE_PASS) {
+		/* the sums of H(1) against W(0), H(0); every iteration then starts with its H update */
+		if (!e)
+			e = one_pass(j, 0, 0, j->st);
+	} else {
+		if (!e)
+			e = nmfk_launch_colsum_parts(j->W, j->n, j->kt, j->colW, j->stream);
+		if (!e)
+			e = nmfk_job_shadow_for_H(j);
+	}
 	if (e)
 		return nmfk_hip_fail(e, "run setup");
-	if (niter_conv > 0 && (e = build_interval_graph(j, T)) != 0)
+	if (niter_conv > 0 && (e = build_interval_graph(j, T, sched)) != 0)
 		return e;
 	e = nmfk_hip_fail(hipStreamSynchronize(j->stream), "run setup");
 	if (e)
@@ -827,6 +931,12 @@ int nmfk_job_run(nmfk_job *j, const nmfk_options *opt, nmfk_result *res)
 	}
 	if (!converged && niter_rem) {
 		for (uint32_t i = 0; i < niter_rem && !e; i++) {
+			if (sched == NMFK_SCHED_ONE_PASS) {
+				e = one_pass_finish(j, j->st);
+				if (!e)
+					e = one_pass(j, 1, 0, j->st);
+				continue;
+			}
 			e = nmfk_enqueue_half_H(j, j->st);
 			if (!e)
 				e = nmfk_enqueue_half_W(j, j->st, 0);
@@ -1004,6 +1114,57 @@ int nmfk_measure_stream(size_t bytes, uint32_t iters, uint32_t blocks, double *r
 		*read_GBps = (double)(nf * 4) / (ms[0] * 1e-3) / 1e9;
 	if (copy_GBps)
 		*copy_GBps = (double)(2 * nf * 4) / (ms[1] * 1e-3) / 1e9;
+	return 0;
+}
+
+static int time_one_pass_loop(nmfk_job *j, uint32_t iters, int kind, double *ms)
+{
+	hipEvent_t a, b;
+	int e = hipEventCreate(&a);
+	if (!e)
+		e = hipEventCreate(&b);
+	if (e)
+		return nmfk_hip_fail(e, "hipEventCreate");
+	e = hipEventRecord(a, j->stream);
+	for (uint32_t i = 0; i < iters && !e; i++) {
+		if (kind != 1)
+			e = one_pass_finish(j, NULL);
+		if (!e && kind != 2)
+			e = one_pass(j, 1, 0, NULL);
+	}
+	if (!e)
+		e = hipEventRecord(b, j->stream);
+	if (!e)
+		e = hipEventSynchronize(b);
+	float f = 0.0f;
+	if (!e)
+		e = hipEventElapsedTime(&f, a, b);
+	hipEventDestroy(a);
+	hipEventDestroy(b);
+	*ms = (double)f / (iters ? iters : 1);
+	return nmfk_hip_fail(e, "timing");
+}
+
+int nmfk_job_time_one_pass(nmfk_job *j, uint32_t iters, nmfk_timing_one_pass *t)
+{
+	if (needs_device_V(j, "nmfk_job_time_one_pass"))
+		return NMFK_EINVAL;
+	if (one_pass_buffers(j))
+		return NMFK_EINVAL;
+	int e = one_pass(j, 0, 0, NULL);
+	if (e)
+		return nmfk_hip_fail(e, "timing setup");
+	double ms[3];
+	for (int kind = 0; kind < 3; kind++) {
+		e = time_one_pass_loop(j, iters, kind, &ms[kind]);
+		if (e)
+			return e;
+	}
+	t->ms_per_iter = ms[0];
+	t->ms_pass = ms[1];
+	t->ms_finish = ms[2];
+	t->workgroups = j->fG;
+	t->waves_per_workgroup = j->fnwv;
 	return 0;
 }
 

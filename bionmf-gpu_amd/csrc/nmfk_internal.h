@@ -46,6 +46,13 @@ struct nmfk_job {
 	hipEvent_t ev[2];
 	hipGraphExec_t chunk_exec;
 	uint32_t chunk_T;
+	int chunk_sched;               /* schedule the captured interval graph runs */
+	/* One pass over V per iteration (nmfk_job_set_schedule): requested schedule, and the
+	 * pass's per-workgroup sums (fG x m x kt) and column sums of W (fG x kt), allocated on
+	 * the first run that uses it. */
+	int sched;
+	float *fpart, *fcolW;
+	uint32_t fG, fnwv;
 	int colW_valid;
 	int vt_stale;                  /* V changed since Vt was built */
 	int owns_V;                    /* 0: V and Vt belong to another job (nmfk_job_create_shared) */

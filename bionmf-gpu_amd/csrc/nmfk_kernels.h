@@ -64,7 +64,8 @@ typedef struct nmfk_hplan {
 #define NMFK_TUNE_MFMA_NA 4      /* MFMA workgroup: waves side by side along the updated factor */
 #define NMFK_TUNE_MFMA_NB 5      /* MFMA workgroup: wave groups splitting the rows of U */
 #define NMFK_TUNE_WGS_BF16 6     /* workgroups the BF16 partial kernel aims for */
-#define NMFK_TUNE_COUNT 7
+#define NMFK_TUNE_FUSED_WAVES 7  /* waves the one-pass kernel aims for (workgroups x waves each) */
+#define NMFK_TUNE_COUNT 8
 int nmfk_tune_set(int key, int value);
 int nmfk_tune_get(int key);
 
@@ -91,6 +92,18 @@ int nmfk_launch_partial(const float *U, size_t ldu, const float *A, const float 
  * then optionally the R_MIN clamp, then colA[blk][c] = column sums of the new A over
  * each block of NMFK_FIN_ROWS rows (the next half-step's denominators).
  */
+/* One pass over V per iteration (k_fused_mfma): eligible shapes and their launch plan.
+ * Returns 0 and the workgroups G and waves per workgroup when the pass can run the job
+ * (kt 16, m <= 1024), else nonzero. */
+int nmfk_fused_plan(uint32_t n, uint32_t m, uint32_t kt, uint32_t *G, uint32_t *nwv);
+/* The pass: new W rows (upd_w) from W and H, then the next H half-step's sums, one slice
+ * of part (G x m x kt) and one row of colW (G x kt) per workgroup; clamp on test passes. */
+int nmfk_launch_fused(const float *V, size_t ldv, float *W, const float *H, uint32_t n, uint32_t m, uint32_t k,
+		      uint32_t kt, uint32_t G, uint32_t nwv, float *part, float *colW, int upd_w, int clamp,
+		      const nmfk_state *st, hipStream_t stream);
+/* H = H * (sum of the G slices) / (sum of the G colW rows), columns < k. */
+int nmfk_launch_fused_finish(const float *part, uint32_t G, const float *colW, float *H, uint32_t m, uint32_t k,
+			     uint32_t kt, const nmfk_state *st, hipStream_t stream);
 int nmfk_launch_finish(const float *part, uint32_t S, float *A, uint32_t p, uint32_t k, uint32_t kt,
 		       const float *den, float *colA, int clamp, const nmfk_state *st,
 		       hipStream_t stream);

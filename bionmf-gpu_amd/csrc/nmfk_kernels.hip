@@ -1178,6 +1178,493 @@ __global__ __launch_bounds__(NMFK_WG) void k_den(const float *__restrict__ parts
 }
 
 // --------------------------------------------------------------------------------------
+// One pass over V per iteration (FP32 matrix cores, 16x16x4).
+//
+// With H(t) fixed, the W half-step of iteration t and the sums of the H half-step of
+// iteration t + 1 both run over the rows of V, and both are row-separable:
+//   W(t)[i]    = W(t-1)[i] * (sum_j H(t)[j] V[i][j] / (W(t-1)[i] . H(t)[j])) / colsum(H(t))
+//   sums[j]   += W(t)[i] V[i][j] / (W(t)[i] . H(t)[j])          over the rows i of V
+// so one workgroup can take a block of 16 whole rows, finish their new W rows, and add
+// their share of the next H update while the block is still on chip: V is read from HBM
+// once per iteration instead of once per half-step (src/NMF_routines.c:704-1000 then
+// :1344-1636, in the same order; the stream path of nmfk_stream.c does the same per block).
+//
+// Workgroup = NWV waves; wave w owns the 16 JT columns j0 = 16 JT w .. of V as JT tiles jt
+// of 16, and holds H at those rows in registers for the whole pass, in the two operand
+// layouts the products need.  Each phase runs over all the wave's tiles without branches
+// (tiles past m hold zero rows of H and are masked), so the tiles' independent matrix-core
+// chains interleave.  For each row block (rows i0 .. i0 + 15):
+//   phase A (W rows):  GEMM1 S[j][i] = H[j] . W[i] (M = j, N = i, K = rank), ratio
+//            R = V / S with V[i0 + l16][j0 + 16jt + 4g + r] (one float4 per lane and tile),
+//            GEMM2 Waux[i][c] += sum_j R[j][i] H[j][c] (K = j: GEMM1's output register r is
+//            the A operand of the K chunk {4g + r}, so no data moves between the products);
+//            the waves' Waux are added through LDS in wave order, and the new rows
+//            W[i] * Waux[i] / colsum(H) (multiply, then divide) are written out;
+//   phase B (H sums):  GEMM1 S'[i][j] = W_new[i] . H[j] (M = i, N = j), ratio with
+//            V[i0 + 4g + r][j0 + 16jt + l16], i.e. the block transposed, which the wave
+//            reads back from its own LDS copy of the tiles it loaded in phase A; GEMM2
+//            sums[j][c] += sum_i R'[i][j] W_new[i][c] (K = i) in registers across blocks.
+// A workgroup takes a contiguous range of row blocks and writes its sums as one slice of
+// part (G x m x KT) plus the column sums of its new W rows; k_fused_finish adds the slices
+// in workgroup order.  colsum(H(t)) is formed in the kernel from the H it holds.
+//
+// Test passes (the last iteration of a test interval) clamp the new W rows to R_MIN before
+// phase B and contract phase B against R_MIN-clamped H, as the reference clamps both before
+// its test (src/NMF_GPU.c:611-638) after update_W used the unclamped H; the test kernel then
+// clamps H in place before k_fused_finish multiplies it.
+// --------------------------------------------------------------------------------------
+
+// x where ok, else +0 (a bit mask, not a branch: the operands are always computed, and the
+// compiler does not move their loads behind a branch).
+__device__ __forceinline__ float masked(bool ok, float x)
+{
+	return __uint_as_float(__float_as_uint(x) & (ok ? 0xffffffffu : 0u));
+}
+
+#define NMFK_FUSED_RB 16      // rows of V per block
+#define NMFK_FUSED_VRS 20     // LDS row stride (floats) of a wave's 16 x 16 V tiles
+#define NMFK_FUSED_COLS 128   // columns of V per wave (8 tiles)
+#define NMFK_FUSED_MAXW 8     // waves per workgroup at most (m <= 1024)
+
+struct fused_args {
+	const float *V;
+	size_t ldv;
+	float *W;             // n x KT: W(t-1) in, W(t) out, rows of this workgroup only
+	const float *H;       // m x KT: H(t), read only
+	float *part;          // G x m x KT: this pass's sums of the next H half-step
+	float *colW;          // G x KT: column sums of the new W rows of each workgroup
+	int n, m, k;
+	int nblocks;          // ceil(n / 16)
+	int G;                // workgroups; workgroup g takes blocks [g nblocks / G, (g+1) nblocks / G)
+	int upd_w;            // 0: W stays (the first pass only adds the sums of H(1) against W(0))
+	int clamp;            // test pass
+};
+
+template <int KT, int NWV, int JT>
+__global__ __launch_bounds__(64 * NWV, 2) void k_fused_mfma(fused_args fa, const nmfk_state *__restrict__ st)
+{
+	static_assert(KT % 16 == 0, "matrix-core pass needs KT % 16 == 0");
+	static_assert(NWV * 64 >= NMFK_FUSED_RB * KT, "one thread per new W element of a block");
+	constexpr int HN = KT / 16;   // 16-column groups of the rank
+	constexpr int KS = KT / 4;    // K steps of GEMM1 (K = the rank, 4 per MFMA)
+	constexpr int NA = 1;         // phase A accumulators (tiles jt add into accA[jt % NA])
+	constexpr int RB = NMFK_FUSED_RB;
+	constexpr int VRS = NMFK_FUSED_VRS;
+	__shared__ float red[NWV][RB][KT];          // per-wave W-half sums of the block
+	__shared__ float wnl[RB][KT + 1];           // the block's new W rows
+	__shared__ float wol[RB][KT + 1];           // the block's W(t-1) rows
+	__shared__ float cws[RB][KT + 1];           // end: column sums of the new W, by row slot
+	__shared__ float hsum[NWV][KT];             // per-wave column sums of H(t)
+	__shared__ float den_s[KT];                 // colsum(H(t))
+	__shared__ __align__(16) float vt[NWV][JT][RB][VRS];   // each wave's V tiles, for the transposed read
+	__shared__ __align__(16) float hbl[NWV][JT][64][4 * HN]; // each lane's GEMM2 operand of H in phase A
+	if (st != nullptr && st->converged)
+		return;
+
+	const int lane = threadIdx.x & 63;
+	const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+	const int l16 = lane & 15, g = lane >> 4;
+	const int n = fa.n, m = fa.m, k = fa.k;
+	const size_t ldv = fa.ldv;
+	const int j0 = wave * 16 * JT;
+	const bool active = j0 < m;               // wave-uniform: the wave has columns
+	const int gid = blockIdx.x;
+	const int b0 = (int)(((long long)gid * fa.nblocks) / fa.G);
+	const int b1 = (int)(((long long)(gid + 1) * fa.nblocks) / fa.G);
+	const float *__restrict__ H = fa.H;
+	const int t = threadIdx.x;
+	const int wi = t / KT, wc = t % KT;       // W element of the block this thread finishes (t < RB*KT)
+	const bool wown = t < RB * KT;
+
+	// Loads of a block, issued one block ahead of its phase A: V rows through a buffer
+	// descriptor based at the block's first row and sized to the rows that exist (rows past n
+	// read zero; offsets past a row's columns read other rows or zero, and are masked), and
+	// W(t-1) rows likewise: wa[s] = W[i0 + l16][4s + g], GEMM1's N operand in phase A, which
+	// wave 0 also leaves in LDS (wol) for the new-row step.  Nothing reads these registers
+	// before phase A, so their loads stay in flight across the barriers in between.
+	const int voff = (int)(((size_t)l16 * ldv + (size_t)(j0 + 4 * g)) * 4);   // tile jt: + 64 jt bytes
+	float4 vr[JT];
+	float wa[KS];
+	auto rows_of = [&](int blk) { return min(RB, n - blk * RB); };
+	auto load_blk = [&](int blk) {
+		const int i0 = blk * RB;
+		const int rows = rows_of(blk);
+		const __amdgpu_buffer_rsrc_t rv = make_rsrc(fa.V + (size_t)i0 * ldv, (size_t)rows * ldv * 4);
+#pragma unroll
+		for (int jt = 0; jt < JT; jt++)
+			vr[jt] = bload4(rv, voff + 64 * jt, 0);
+		const __amdgpu_buffer_rsrc_t rw = make_rsrc(fa.W + (size_t)i0 * KT, (size_t)rows * KT * 4);
+#pragma unroll
+		for (int s = 0; s < KS; s++)
+			wa[s] = bload1(rw, (l16 * KT + 4 * s + g) * 4, 0);
+	};
+	auto zero4 = [](floatx4 (&a)[HN]) {
+#pragma unroll
+		for (int h = 0; h < HN; h++)
+			a[h] = floatx4{0.f, 0.f, 0.f, 0.f};
+	};
+	// The first block's loads are in flight while H is set up.
+	if (b0 < b1)
+		load_blk(b0);
+
+	// H (m x KT, contiguous) is staged in LDS through the V-tile area, which is free until the
+	// first phase A, with coalesced 16-byte loads by the whole workgroup.  From there each
+	// wave takes H at its columns, in registers: ha[jt][s] = H[j0 + 16jt + l16][4s + g] (the
+	// K = rank operand of both phases), and in the lane's own LDS slots:
+	// hbl[w][jt][lane][4h + r] = H[j0 + 16jt + 4g + r][16h + l16] (phase A's K = j operand).
+	// Rows past m are zero, so past-the-end tiles contribute nothing.
+	static_assert(sizeof(vt) >= (size_t)NWV * JT * 16 * KT * sizeof(float), "H staging fits the V tiles");
+	float *hs = &vt[0][0][0][0];
+	{
+		const int nq = m * KT / 4;   // float4s of H
+		const float4 *H4 = reinterpret_cast<const float4 *>(H);
+		for (int q = t; q < nq; q += 64 * NWV)
+			*reinterpret_cast<float4 *>(hs + 4 * q) = H4[q];
+	}
+	__syncthreads();
+	float ha[JT][KS];
+	float hcs[HN];
+#pragma unroll
+	for (int h = 0; h < HN; h++)
+		hcs[h] = 0.0f;
+#pragma unroll
+	for (int jt = 0; jt < JT; jt++) {
+		const int ja = j0 + 16 * jt + l16;
+#pragma unroll
+		for (int s = 0; s < KS; s++)
+			ha[jt][s] = ja < m ? hs[ja * KT + 4 * s + g] : 0.0f;
+#pragma unroll
+		for (int h = 0; h < HN; h++) {
+			float4 q;
+			float *qv = reinterpret_cast<float *>(&q);
+#pragma unroll
+			for (int r = 0; r < 4; r++) {
+				const int jb = j0 + 16 * jt + 4 * g + r;
+				qv[r] = jb < m ? hs[jb * KT + 16 * h + l16] : 0.0f;
+				hcs[h] += qv[r];
+			}
+			*reinterpret_cast<float4 *>(&hbl[wave][jt][lane][4 * h]) = q;
+		}
+	}
+	// colsum(H(t)): the wave's rows in a fixed order (tiles, rows r, then the lane groups by
+	// a xor tree), then the waves in order.
+#pragma unroll
+	for (int h = 0; h < HN; h++) {
+		float s = hcs[h];
+		s += __shfl_xor(s, 16, 64);
+		s += __shfl_xor(s, 32, 64);
+		if (g == 0)
+			hsum[wave][16 * h + l16] = s;
+	}
+	__syncthreads();   // also: every wave has read hs before the V tiles overwrite it
+	if (t < KT) {
+		float s = 0.0f;
+#pragma unroll
+		for (int w = 0; w < NWV; w++)
+			s += hsum[w][t];
+		den_s[t] = s;
+	}
+	// (den_s is read after the first barrier of the block loop)
+
+	// Phase B contracts against H clamped to R_MIN on test passes (columns < k only).
+
+	floatx4 accB[JT][HN];
+#pragma unroll
+	for (int jt = 0; jt < JT; jt++)
+#pragma unroll
+		for (int h = 0; h < HN; h++)
+			accB[jt][h] = floatx4{0.f, 0.f, 0.f, 0.f};
+	float cw = 0.0f;
+
+	// Each phase is two straight-line stages over all tiles, so that the tiles' independent
+	// matrix chains interleave: GEMM1, then the ratios and GEMM2.  The loop body orders them
+	// GEMM1(B), GEMM1(A), ratios + GEMM2(B), ratios + GEMM2(A), so that each phase's ratios
+	// (vector ALU work) run while the other phase's products occupy the matrix cores.
+	// Phase A, GEMM1: S[jt][r] = S[j0 + 16jt + 4g + r][i0 + l16] = H[j] . W(t-1)[i].
+	auto gemm1_a = [&](floatx4 (&S)[JT]) {
+#pragma unroll
+		for (int jt = 0; jt < JT; jt++)
+			S[jt] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+		for (int s = 0; s < KS; s++)
+#pragma unroll
+			for (int jt = 0; jt < JT; jt++)
+				S[jt] = __builtin_amdgcn_mfma_f32_16x16x4f32(ha[jt][s], wa[s], S[jt], 0, 0, 0);
+	};
+	// Phase A, ratios and GEMM2: accA[jt % NA][h][q] = Waux[i0 + 4g + q][16h + l16] over tile jt.
+	// FULL: the block has 16 rows and the wave's columns are all < m (no masks).
+	auto gemm2_a = [&](int rows, const floatx4 (&S)[JT], floatx4 (&accA)[NA][HN], auto full_tag) {
+		constexpr bool FULL = decltype(full_tag)::value;
+		const bool iok = l16 < rows;
+#pragma unroll
+		for (int jt = 0; jt < JT; jt++) {
+			const float v[4] = {vr[jt].x, vr[jt].y, vr[jt].z, vr[jt].w};
+			float R[4];
+#pragma unroll
+			for (int r = 0; r < 4; r++) {
+				const float x = v[r] * __builtin_amdgcn_rcpf(S[jt][r]);
+				R[r] = FULL ? x : masked(iok && j0 + 16 * jt + 4 * g + r < m, x);
+			}
+#pragma unroll
+			for (int h = 0; h < HN; h++) {
+				const float4 q = *reinterpret_cast<const float4 *>(&hbl[wave][jt][lane][4 * h]);
+				const float hv[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+				for (int r = 0; r < 4; r++)
+					accA[jt % NA][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(R[r], hv[r], accA[jt % NA][h], 0, 0, 0);
+			}
+		}
+	};
+	// The V tiles a wave keeps for phase B (vt[w][jt][row][col] = V[i0 + row][j0 + 16jt + col]),
+	// and wave 0's copy of the W(t-1) rows for the new-row step.
+	auto store_vt = [&]() {
+#pragma unroll
+		for (int jt = 0; jt < JT; jt++)
+			*reinterpret_cast<float4 *>(&vt[wave][jt][l16][4 * g]) = vr[jt];
+		if (wave == 0)
+#pragma unroll
+			for (int s = 0; s < KS; s++)
+				wol[l16][4 * s + g] = wa[s];
+	};
+	// The wave's W-half sums of the block.
+	auto store_red = [&](const floatx4 (&accA)[NA][HN]) {
+#pragma unroll
+		for (int h = 0; h < HN; h++)
+#pragma unroll
+			for (int q = 0; q < 4; q++)
+				red[wave][4 * g + q][16 * h + l16] = accA[0][h][q];
+	};
+	// Phase B (the next H half-step's sums against the new rows), GEMM1:
+	// S[jt][r] = S'[i0 + 4g + r][j0 + 16jt + l16] = W(t)[i] . H[j], H clamped on test passes.
+	auto gemm1_b = [&](const float (&wna)[KS], floatx4 (&S)[JT], auto clamp_tag) {
+		constexpr bool CLAMP = decltype(clamp_tag)::value;
+#pragma unroll
+		for (int jt = 0; jt < JT; jt++)
+			S[jt] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+		for (int s = 0; s < KS; s++)
+#pragma unroll
+			for (int jt = 0; jt < JT; jt++) {
+				const float hv = (CLAMP && 4 * s + g < k) ? clamp_rmin(ha[jt][s]) : ha[jt][s];
+				S[jt] = __builtin_amdgcn_mfma_f32_16x16x4f32(wna[s], hv, S[jt], 0, 0, 0);
+			}
+	};
+	// Phase B, ratios (V read back transposed from the wave's LDS tiles) and GEMM2.
+	auto gemm2_b = [&](int rows, const floatx4 (&S)[JT], const float (&wnb)[4][HN], auto full_tag) {
+		constexpr bool FULL = decltype(full_tag)::value;
+#pragma unroll
+		for (int jt = 0; jt < JT; jt++) {
+			const bool jok = j0 + 16 * jt + l16 < m;
+			float v[4], R[4];
+#pragma unroll
+			for (int r = 0; r < 4; r++)
+				v[r] = vt[wave][jt][4 * g + r][l16];
+#pragma unroll
+			for (int r = 0; r < 4; r++) {
+				const float x = v[r] * __builtin_amdgcn_rcpf(S[jt][r]);
+				R[r] = FULL ? x : masked(jok && 4 * g + r < rows, x);
+			}
+#pragma unroll
+			for (int r = 0; r < 4; r++)
+#pragma unroll
+				for (int h = 0; h < HN; h++)
+					accB[jt][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(R[r], wnb[r][h], accB[jt][h], 0, 0, 0);
+		}
+	};
+	// One loop body: phase B of block b and (WITH_A) phase A of block b + 1, interleaved;
+	// FULL when both blocks have 16 rows and the wave's columns are all < m, CLAMP on test
+	// passes (compiled apart, so the common body carries no masks and no clamp).
+	auto block_body = [&](int rows, int rows_n, floatx4 (&accA)[NA][HN], auto with_a_tag, auto full_tag,
+			      auto clamp_tag) {
+		constexpr bool WITH_A = decltype(with_a_tag)::value;
+		float wna[KS], wnb[4][HN];
+#pragma unroll
+		for (int s = 0; s < KS; s++)
+			wna[s] = wnl[l16][4 * s + g];
+#pragma unroll
+		for (int r = 0; r < 4; r++)
+#pragma unroll
+			for (int h = 0; h < HN; h++)
+				wnb[r][h] = wnl[4 * g + r][16 * h + l16];
+		floatx4 SB[JT], SA[JT];
+		gemm1_b(wna, SB, clamp_tag);
+		if constexpr (WITH_A)
+			gemm1_a(SA);
+		gemm2_b(rows, SB, wnb, full_tag);
+		if constexpr (WITH_A)
+			gemm2_a(rows_n, SA, accA, full_tag);
+	};
+	const bool cols_full = j0 + 16 * JT <= m;   // wave-uniform
+
+	// Software pipeline over the workgroup's row blocks: the loop body finishes block b's new
+	// rows, runs phase B of block b, then phase A of block b + 1 (whose V tiles replace block
+	// b's in LDS once phase B has read them), while the loads of block b + 2 are in flight.
+	// Wave 0 is always active (m >= 1), so wol is written for every block.
+	if (b0 < b1) {
+		if (active) {
+			floatx4 accA[NA][HN];
+#pragma unroll
+			for (int a = 0; a < NA; a++)
+				zero4(accA[a]);
+			if (fa.upd_w) {
+				floatx4 SA[JT];
+				gemm1_a(SA);
+				gemm2_a(rows_of(b0), SA, accA, std::false_type{});
+			}
+			store_vt();
+			store_red(accA);
+		} else {
+			floatx4 accA[NA][HN];
+#pragma unroll
+			for (int a = 0; a < NA; a++)
+				zero4(accA[a]);
+			store_red(accA);
+		}
+		if (b0 + 1 < b1)
+			load_blk(b0 + 1);
+	}
+
+	for (int b = b0; b < b1; b++) {
+		const int i0 = b * RB;
+		const int rows = rows_of(b);
+		__syncthreads();
+		// New W rows of block b (kernel_mul_div order: multiply, then divide).
+		if (wown) {
+			const float wold = wol[wi][wc];
+			float wn = 0.0f;
+			if (wi < rows) {
+				if (fa.upd_w) {
+					if (wc < k) {
+						float waux = 0.0f;
+#pragma unroll
+						for (int w = 0; w < NWV; w++)
+							waux += red[w][wi][wc];
+						wn = wold * waux;
+						wn = wn / den_s[wc];
+						if (fa.clamp)
+							wn = clamp_rmin(wn);
+					}
+					fa.W[(size_t)(i0 + wi) * KT + wc] = wn;
+				} else {
+					wn = wold;
+				}
+			}
+			wnl[wi][wc] = wn;
+			cw += wn;
+		}
+		__syncthreads();
+		const bool next = b + 1 < b1;
+		floatx4 accA[NA][HN];
+#pragma unroll
+		for (int a = 0; a < NA; a++)
+			zero4(accA[a]);
+		if (active) {
+			const std::true_type T;
+			const std::false_type F;
+			if (next && fa.upd_w) {
+				const bool full = cols_full && rows == RB && rows_of(b + 1) == RB;
+				if (full && !fa.clamp)
+					block_body(rows, RB, accA, T, T, F);
+				else if (full)
+					block_body(rows, RB, accA, T, T, T);
+				else if (!fa.clamp)
+					block_body(rows, rows_of(b + 1), accA, T, F, F);
+				else
+					block_body(rows, rows_of(b + 1), accA, T, F, T);
+			} else if (fa.clamp) {
+				block_body(rows, 0, accA, F, F, T);
+			} else {
+				block_body(rows, 0, accA, F, F, F);
+			}
+			if (next)
+				store_vt();
+		}
+		if (next)
+			store_red(accA);
+		if (b + 2 < b1)
+			load_blk(b + 2);
+		// red and wnl are rewritten only after the next barrier, when every wave has read them.
+	}
+
+	// This workgroup's slice of the sums: accB[jt][h][q] = sums[j0 + 16jt + 4g + q][16h + l16].
+	float *dst = fa.part + (size_t)gid * m * KT;
+#pragma unroll
+	for (int jt = 0; jt < JT; jt++)
+#pragma unroll
+		for (int q = 0; q < 4; q++) {
+			const int j = j0 + 16 * jt + 4 * g + q;
+			if (j < m)
+#pragma unroll
+				for (int h = 0; h < HN; h++)
+					dst[(size_t)j * KT + 16 * h + l16] = accB[jt][h][q];
+		}
+	if (wown)
+		cws[wi][wc] = cw;
+	__syncthreads();
+	if (t < KT) {
+		float s = 0.0f;
+#pragma unroll
+		for (int i = 0; i < RB; i++)
+			s += cws[i][t];
+		fa.colW[(size_t)gid * KT + t] = s;
+	}
+}
+
+// H(t+1) = H(t) * (sum over the G slices of part, in slice order) / colsum(W(t)), with
+// colsum(W(t)) from the G column-sum rows (den_reduce).  One output element per lane; the
+// 16 waves of a workgroup take the slices g = w, w + 16, ... and are added in wave order.
+#define NMFK_FFIN_WAVES 16
+
+template <int KT>
+__global__ __launch_bounds__(64 * NMFK_FFIN_WAVES) void k_fused_finish(const float *__restrict__ part, int G,
+									 const float *__restrict__ colW,
+									 float *__restrict__ H, int m, int k,
+									 const nmfk_state *__restrict__ st)
+{
+	__shared__ float red[NMFK_FFIN_WAVES][64];
+	__shared__ float s4[4 * KT];
+	__shared__ float den[KT];
+	if (st != nullptr && st->converged)
+		return;
+	den_reduce<KT>(colW, G, den, s4);
+	const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+	const size_t tot = (size_t)m * KT;
+	const size_t e = (size_t)blockIdx.x * 64 + lane;
+	float s = 0.0f;
+	if (e < tot) {
+		const float *p = part + e;
+		int gg = w;
+		// 8 slices' loads in flight per step; the additions keep ascending slice order.
+		for (; gg + 7 * NMFK_FFIN_WAVES < G; gg += 8 * NMFK_FFIN_WAVES) {
+			float v[8];
+#pragma unroll
+			for (int u = 0; u < 8; u++)
+				v[u] = p[(size_t)(gg + u * NMFK_FFIN_WAVES) * tot];
+#pragma unroll
+			for (int u = 0; u < 8; u++)
+				s += v[u];
+		}
+		for (; gg < G; gg += NMFK_FFIN_WAVES)
+			s += p[(size_t)gg * tot];
+	}
+	red[w][lane] = s;
+	__syncthreads();
+	if (w == 0 && e < tot) {
+		float sum = 0.0f;
+#pragma unroll
+		for (int u = 0; u < NMFK_FFIN_WAVES; u++)
+			sum += red[u][lane];
+		const int c = (int)(e % KT);
+		float v = 0.0f;
+		if (c < k) {
+			v = H[e] * sum;
+			v = v / den[c];
+		}
+		H[e] = v;
+	}
+}
+
+// --------------------------------------------------------------------------------------
 // Argmax with the reference's thread layout (src/GPU_kernels.cu:2592-2830,
 // src/matrix_operations.c:190-203): bw "threads" (lanes) each scan columns t, t+bw, t+2bw
 // with a strict '>' starting from their first column (or from (-1, 0) without one); a
@@ -1788,7 +2275,7 @@ static inline uint32_t default_ap(uint32_t kt)
 // one four-wave workgroup per CU as well as with two.  At 50000 x 2000, k = 64 its H half
 // takes 101 us with the 8 splits a 256-workgroup target gives, 105 us with 16 (512) and
 // 115 us with the 32 of the FP32 target; 12 splits (384) take 127 us.
-static int g_tune[NMFK_TUNE_COUNT] = {512, 1024, 8, 64, 1, 4, 256};
+static int g_tune[NMFK_TUNE_COUNT] = {512, 1024, 8, 64, 1, 4, 256, 2048};
 
 static bool mfma_shape_ok(int na, int nb)
 {
@@ -1989,6 +2476,54 @@ extern "C" int nmfk_launch_finish_shadow(const float *part, uint32_t S, float *A
 	default:
 		return (int)hipErrorInvalidValue;
 	}
+}
+
+extern "C" int nmfk_fused_plan(uint32_t n, uint32_t m, uint32_t kt, uint32_t *G, uint32_t *nwv)
+{
+	if (kt != 16 || n == 0 || m == 0 || m > NMFK_FUSED_MAXW * NMFK_FUSED_COLS)
+		return -1;
+	const uint32_t nw = m <= 4 * NMFK_FUSED_COLS ? 4 : 8;
+	const uint32_t nb = (n + NMFK_FUSED_RB - 1) / NMFK_FUSED_RB;
+	const uint32_t target = std::max(1u, (uint32_t)g_tune[NMFK_TUNE_FUSED_WAVES] / nw);
+	const uint32_t bpw = (nb + target - 1) / target;   // row blocks per workgroup, balanced
+	*G = (nb + bpw - 1) / bpw;
+	*nwv = nw;
+	return 0;
+}
+
+extern "C" int nmfk_launch_fused(const float *V, size_t ldv, float *W, const float *H, uint32_t n, uint32_t m,
+				 uint32_t k, uint32_t kt, uint32_t G, uint32_t nwv, float *part, float *colW, int upd_w,
+				 int clamp, const nmfk_state *st, hipStream_t stream)
+{
+	if (kt != 16 || ldv < m || ldv % 4 != 0 || G == 0)
+		return (int)hipErrorInvalidValue;
+	fused_args fa;
+	fa.V = V, fa.ldv = ldv, fa.W = W, fa.H = H, fa.part = part, fa.colW = colW;
+	fa.n = (int)n, fa.m = (int)m, fa.k = (int)k;
+	fa.nblocks = (int)((n + NMFK_FUSED_RB - 1) / NMFK_FUSED_RB);
+	fa.G = (int)G, fa.upd_w = upd_w, fa.clamp = clamp;
+	if (m > nwv * NMFK_FUSED_COLS)
+		return (int)hipErrorInvalidValue;
+	constexpr int JT = NMFK_FUSED_COLS / 16;
+	if (nwv == 4)
+		hipLaunchKernelGGL((k_fused_mfma<16, 4, JT>), dim3(G), dim3(64 * 4), 0, stream, fa, st);
+	else if (nwv == 8)
+		hipLaunchKernelGGL((k_fused_mfma<16, 8, JT>), dim3(G), dim3(64 * 8), 0, stream, fa, st);
+	else
+		return (int)hipErrorInvalidValue;
+	return (int)hipGetLastError();
+	return (int)hipGetLastError();
+}
+
+extern "C" int nmfk_launch_fused_finish(const float *part, uint32_t G, const float *colW, float *H, uint32_t m,
+					uint32_t k, uint32_t kt, const nmfk_state *st, hipStream_t stream)
+{
+	if (kt != 16)
+		return (int)hipErrorInvalidValue;
+	const unsigned blocks = (unsigned)(((size_t)m * kt + 63) / 64);
+	hipLaunchKernelGGL((k_fused_finish<16>), dim3(blocks), dim3(64 * NMFK_FFIN_WAVES), 0, stream, part, (int)G,
+			   colW, H, (int)m, (int)k, st);
+	return (int)hipGetLastError();
 }
 
 extern "C" int nmfk_launch_finish(const float *part, uint32_t S, float *A, uint32_t p, uint32_t k, uint32_t kt,

@@ -124,6 +124,23 @@ int nmfk_job_sync(nmfk_job *job);
 int nmfk_job_set_kernel_path(nmfk_job *job, int path);
 int nmfk_job_kernel_path(const nmfk_job *job);
 
+/* Iteration schedule of nmfk_job_run.  NMFK_SCHED_TWO_PASS: one pass over V per half-step
+ * (V for the H half, its transposed copy for the W half), the schedule batches and the
+ * drop-in half-steps use.  NMFK_SCHED_ONE_PASS: one pass over V per iteration, each row
+ * block giving its new W rows and then its share of the next H half-step's sums (FP32
+ * matrix cores; padded rank 16, i.e. k 9..16, and m <= 1024 on the default kernel path); it
+ * reads V half as often but does the same matrix-core work, on which both are bound at these
+ * ranks.  NMFK_SCHED_AUTO (default): two passes.  Both keep the reference's update order,
+ * clamp and stop rule; their sums are added in different orders, so they agree to rounding,
+ * not bit for bit. */
+#define NMFK_SCHED_AUTO 0
+#define NMFK_SCHED_TWO_PASS 1
+#define NMFK_SCHED_ONE_PASS 2
+int nmfk_job_set_schedule(nmfk_job *job, int sched);
+/* The schedule nmfk_job_run will use (NMFK_SCHED_TWO_PASS or NMFK_SCHED_ONE_PASS; a requested
+ * one pass reads as two where the job's kernel path no longer allows it). */
+int nmfk_job_schedule(const nmfk_job *job);
+
 /* Launch-shape tunables of the kernel planner, process-wide; they apply to jobs created
  * afterwards.  Keys: 0 = workgroups the vector-ALU partial kernel aims for (default 512),
  * 1 = workgroups the FP32 matrix-core partial kernel aims for (default 1024), 2 = minimum
@@ -131,7 +148,8 @@ int nmfk_job_kernel_path(const nmfk_job *job);
  * splits of the contraction range (default 64), 4 and 5 = matrix-core workgroup shape:
  * waves side by side along the updated factor x wave groups splitting the contraction
  * range (default 1 x 4; also 2x2, 4x1, 2x4, 4x2), 6 = workgroups the BF16 matrix-core
- * partial kernel aims for (default 256). */
+ * partial kernel aims for (default 256), 7 = waves the one-pass kernel aims for (workgroups x
+ * waves per workgroup, default 4096). */
 int nmfk_tune_set(int key, int value);
 int nmfk_tune_get(int key);
 
@@ -328,6 +346,16 @@ typedef struct nmfk_timing {
 	double ms_partial_H, ms_finish_H, ms_partial_W, ms_finish_W;
 } nmfk_timing;
 int nmfk_job_time_iterations(nmfk_job *job, uint32_t iters, nmfk_timing *t);
+
+/* Times <iters> iterations of the one-pass schedule (finish of H, then the pass over V; no
+ * tests) with HIP events on the job's stream, after one untimed pass; W and H advance. */
+typedef struct nmfk_timing_one_pass {
+	double ms_per_iter;
+	double ms_pass;          /* the pass over V (new W rows and the next H half's sums) */
+	double ms_finish;        /* the H update from the pass's sums */
+	uint32_t workgroups, waves_per_workgroup;
+} nmfk_timing_one_pass;
+int nmfk_job_time_one_pass(nmfk_job *job, uint32_t iters, nmfk_timing_one_pass *t);
 
 /* Measured streaming ceiling of the current device: a plain float4 read kernel and a
  * copy kernel over a fresh <bytes> buffer, <iters> timed launches of <blocks> workgroups

@@ -7,7 +7,7 @@ import ctypes as C
 import pytest
 
 PATH_AUTO, PATH_VALU, PATH_MFMA32, PATH_BF16 = 0, 1, 2, 3
-DEFAULTS = {0: 512, 1: 1024, 2: 8, 3: 64, 4: 1, 5: 4, 6: 256}
+DEFAULTS = {0: 512, 1: 1024, 2: 8, 3: 64, 4: 1, 5: 4, 6: 256, 7: 2048}
 
 
 class HPlan(C.Structure):
@@ -33,8 +33,8 @@ def plan(lib, p, q, kt, path=PATH_AUTO):
 
 def test_tunable_defaults_and_bounds(lib):
     assert {key: lib.nmfk_tune_get(key) for key in DEFAULTS} == DEFAULTS
-    assert lib.nmfk_tune_get(7) == -1 and lib.nmfk_tune_get(-1) == -1
-    assert lib.nmfk_tune_set(7, 1) != 0 and lib.nmfk_tune_set(1, 0) != 0
+    assert lib.nmfk_tune_get(8) == -1 and lib.nmfk_tune_get(-1) == -1
+    assert lib.nmfk_tune_set(8, 1) != 0 and lib.nmfk_tune_set(1, 0) != 0
     assert lib.nmfk_tune_get(1) == 1024
 
 
@@ -108,3 +108,25 @@ def test_tunables_move_the_plan(lib):
     assert lib.nmfk_tune_set(4, 3) == 0                     # 3 x 2 is no kernel shape: default 1 x 4
     pl = plan(lib, 1000, 20000, 16)
     assert (pl.ap, pl.nw) == (1, 4)
+
+
+@pytest.mark.parametrize("n,m,kt,ok", [(20000, 1000, 16, True), (20000, 1024, 16, True), (20000, 1025, 16, False),
+                                       (20000, 1000, 32, False), (20000, 1000, 8, False), (15, 64, 16, True),
+                                       (1, 1, 16, True), (50000, 2000, 64, False), (0, 10, 16, False)])
+def test_one_pass_plan(lib, n, m, kt, ok):
+    # k_fused_mfma takes padded rank 16 and m <= 1024 (up to 8 waves of 128 columns); its
+    # workgroups split the ceil(n / 16) row blocks evenly (row blocks per workgroup differ by
+    # at most one) and aim for 2048 waves in all.
+    G, nwv = C.c_uint32(), C.c_uint32()
+    rc = lib.nmfk_fused_plan(n, m, kt, C.byref(G), C.byref(nwv))
+    assert (rc == 0) == ok
+    if not ok:
+        return
+    nb = (n + 15) // 16
+    assert nwv.value == (4 if m <= 512 else 8)
+    per_wg = -(-nb // (2048 // nwv.value))          # row blocks per workgroup at the wave target
+    assert G.value == -(-nb // per_wg) and 1 <= G.value <= nb
+    per = [((g + 1) * nb) // G.value - (g * nb) // G.value for g in range(G.value)]
+    assert min(per) >= 1 and max(per) - min(per) <= 1
+    if (n, m) == (20000, 1000):
+        assert (G.value, nwv.value) == (250, 8)

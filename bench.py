@@ -327,7 +327,9 @@ def bench_served(pkg, count: int, rate: float) -> dict:
             "latency_s": {"p50": st["p50"], "p99": st["p99"], "mean": st["mean_latency"]},
             "seconds": st["seconds"], "peak_jobs": st["peak_jobs"], "peak_GB": st["peak_bytes"] / 1e9,
             "restarts": int(sum(r[3] for r in reqs)),
-            "mean_iterations_per_restart": float(sum(o["iterations_sum"] for o in out) / sum(r[3] for r in reqs))}
+            "mean_iterations_per_restart": float(sum(o["iterations_sum"] for o in out) / sum(r[3] for r in reqs)),
+            "serving_thread_s": {"step": st["step_s"], "admit": st["admit_s"], "wait_for_device": st["wait_s"],
+                                 "retire": st["retire_s"], "chunks": st["chunks"]}}
 
 
 def main():

@@ -124,7 +124,9 @@ class ServeStats(C.Structure):
     _fields_ = [("seconds", C.c_double), ("requests", C.c_uint32), ("peak_jobs", C.c_uint32),
                 ("requests_per_s", C.c_double), ("factorizations_per_s", C.c_double),
                 ("p50", C.c_double), ("p99", C.c_double), ("mean_latency", C.c_double),
-                ("peak_bytes", C.c_size_t)]
+                ("peak_bytes", C.c_size_t), ("step_s", C.c_double), ("admit_s", C.c_double),
+                ("wait_s", C.c_double), ("retire_s", C.c_double), ("chunks", C.c_uint32),
+                ("pad_", C.c_uint32)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

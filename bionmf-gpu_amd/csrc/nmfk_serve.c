@@ -218,10 +218,17 @@ int nmfk_serve_run_detail(const nmfk_serve_config *cfg, const nmfk_request *req,
 	size_t used = 0, peak = 0;
 	uint32_t active_jobs = 0, peak_jobs = 0, next = 0, finished = 0, first_active = 0;
 	const double t0 = nmfk_now();
+	double t_step = 0.0, t_admit = 0.0, t_wait = 0.0, t_retire = 0.0, tm;
+	uint32_t nchunks = 0;
 	while (!e && finished < nreq) {
 		/* 1. keep the device busy */
-		if (nmfk_batch_active(b) > 0)
+		tm = nmfk_now();
+		if (nmfk_batch_active(b) > 0) {
 			e = nmfk_batch_step(b, cfg->chunks_per_poll);
+			nchunks++;
+		}
+		t_step += nmfk_now() - tm;
+		tm = nmfk_now();
 		/* 2. admit what has arrived and fits, in arrival order */
 		while (!e && next < nreq) {
 			const nmfk_request *q = &req[next];
@@ -257,6 +264,7 @@ int nmfk_serve_run_detail(const nmfk_serve_config *cfg, const nmfk_request *req,
 				peak_jobs = active_jobs;
 			next++;
 		}
+		t_admit += nmfk_now() - tm;
 		if (e)
 			break;
 		if (nmfk_batch_active(b) == 0) {
@@ -266,7 +274,10 @@ int nmfk_serve_run_detail(const nmfk_serve_config *cfg, const nmfk_request *req,
 		}
 		/* 3. retire finished jobs and requests */
 		uint32_t nd = 0;
+		tm = nmfk_now();
 		e = nmfk_batch_poll(b, done, dres, cfg->capacity, &nd);
+		t_wait += nmfk_now() - tm;
+		tm = nmfk_now();
 		for (uint32_t d = 0; d < nd && !e; d++) {
 			uint32_t qi = first_active, ri = 0;
 			int found = 0;
@@ -317,6 +328,7 @@ int nmfk_serve_run_detail(const nmfk_serve_config *cfg, const nmfk_request *req,
 					first_active++;
 			}
 		}
+		t_retire += nmfk_now() - tm;
 	}
 	const double t1 = nmfk_now() - t0;
 	/* The batch first: destroying it waits for its stream, which may still read jobs. */
@@ -344,6 +356,12 @@ int nmfk_serve_run_detail(const nmfk_serve_config *cfg, const nmfk_request *req,
 		stats->mean_latency = finished ? s / finished : 0.0;
 		stats->peak_bytes = peak;
 		stats->peak_jobs = peak_jobs;
+		stats->step_s = t_step;
+		stats->admit_s = t_admit;
+		stats->wait_s = t_wait;
+		stats->retire_s = t_retire;
+		stats->chunks = nchunks;
+		stats->pad_ = 0;
 	}
 	free(rs), free(done), free(dres), free(lat), free(roff);
 	return e;

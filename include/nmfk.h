@@ -307,6 +307,13 @@ typedef struct nmfk_serve_stats {
 	double requests_per_s, factorizations_per_s;
 	double p50, p99, mean_latency;/* request latency, seconds */
 	size_t peak_bytes;            /* largest admitted total of nmfk_request_bytes */
+	/* Where the serving thread's time went, seconds: enqueueing chunks (including tile-table
+	 * rebuilds and graph re-captures when jobs joined or left), admitting requests (job set-up
+	 * and input generation), waiting for the device in nmfk_batch_poll, and retiring finished
+	 * restarts (their FP64 KL pass and copies). */
+	double step_s, admit_s, wait_s, retire_s;
+	uint32_t chunks;              /* nmfk_batch_step calls that enqueued work */
+	uint32_t pad_;
 } nmfk_serve_stats;
 
 void nmfk_serve_default_config(nmfk_serve_config *cfg);

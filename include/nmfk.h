@@ -149,7 +149,7 @@ int nmfk_job_schedule(const nmfk_job *job);
  * waves side by side along the updated factor x wave groups splitting the contraction
  * range (default 1 x 4; also 2x2, 4x1, 2x4, 4x2), 6 = workgroups the BF16 matrix-core
  * partial kernel aims for (default 256), 7 = waves the one-pass kernel aims for (workgroups x
- * waves per workgroup, default 4096). */
+ * waves per workgroup, default 2048). */
 int nmfk_tune_set(int key, int value);
 int nmfk_tune_get(int key);
 

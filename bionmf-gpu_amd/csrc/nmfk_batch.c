@@ -5,9 +5,10 @@
  * holds up to <cap> jobs of uneven shape (n, m, k) in slots and advances all of them with
  * one launch per phase: partial and finish of the H half, partial and finish of the W
  * half, and every niter_test_conv iterations the test (two launches) and a gather of the
- * 32-byte stop-rule states.  A launch walks a table of tiles; a tile is exactly one
- * workgroup of the job's own single-job grid, run by the same device code, so each job's
- * results are bitwise those of nmfk_job_run on the same job.
+ * 32-byte stop-rule states, which are copied to pinned memory after the chunk.  A launch
+ * walks a table of tiles; a tile is exactly one workgroup of the job's own single-job grid,
+ * run by the same device code, so each job's results are bitwise those of nmfk_job_run on
+ * the same job.
  *
  * Jobs join (nmfk_batch_add) and leave (nmfk_batch_poll) between chunks.  A finished job's
  * tiles stay in the tables as no-ops (its slot is marked empty on the device) until the
@@ -425,9 +426,6 @@ static int enqueue_chunk(nmfk_batch *b)
 					   b->tune.grid_finish, b->stream);
 	if (!e)
 		e = nmfk_launch_batch_gather_states(b->d_desc, b->nslots, b->d_states, b->stream);
-	if (!e)
-		e = hipMemcpyAsync(b->h_states, b->d_states, b->nslots * sizeof(nmfk_state), hipMemcpyDeviceToHost,
-				   b->stream);
 	return e;
 }
 
@@ -467,6 +465,12 @@ int nmfk_batch_step(nmfk_batch *b, uint32_t chunks)
 	}
 	for (uint32_t c = 0; c < chunks && !e; c++)
 		e = hipck(hipGraphLaunch(b->exec, b->stream), "batch chunk");
+	/* The last chunk's gathered states to pinned memory, after the graph rather than as a
+	 * node in it, as the single-job loop copies its state: the chunk graph holds kernels
+	 * only.  A batch whose slots grew was re-captured above, so the gather covers nslots. */
+	if (!e && chunks)
+		e = hipck(hipMemcpyAsync(b->h_states, b->d_states, b->nslots * sizeof(nmfk_state),
+					 hipMemcpyDeviceToHost, b->stream), "batch states");
 	if (!e)
 		e = hipck(hipEventRecord(b->ev, b->stream), "batch event");
 	if (!e && chunks)

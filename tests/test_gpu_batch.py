@@ -64,6 +64,22 @@ def test_batch_equals_jobs_alone(nmfk, orc):
         a.close(), b.close()
 
 
+def test_batch_several_chunks_per_poll(nmfk, orc):
+    # Three chunks per step: the states reach the host once, after the last of them; jobs that
+    # stop in an earlier chunk stay stopped (their tiles return at once), so every result is
+    # still the job's result alone, leftover-iteration jobs included.
+    pairs = [twins(nmfk, orc, s) for s in MIX]
+    batch = nmfk.Batch(T, 16)
+    for (a, _), s in zip(pairs, MIX):
+        batch.add(a, s[5], s[6])
+    done = dict((id(j), r) for j, r in batch.run(chunks_per_poll=3))
+    assert len(done) == len(MIX)
+    for (a, b), s in zip(pairs, MIX):
+        assert_same(a, b, done[id(a)], b.run(s[5], T, s[6]), s)
+    for a, b in pairs:
+        a.close(), b.close()
+
+
 def test_batch_admission_mid_run(nmfk, orc):
     # Jobs arrive while others are running (continuous batching): each result is still the
     # job's result alone.

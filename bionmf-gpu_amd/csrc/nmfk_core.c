@@ -72,8 +72,7 @@ typedef struct job_layout {
 	size_t ldv, ldvt;
 	nmfk_hplan planH, planW;
 	size_t n_V, n_Vt, n_W, n_H, n_part, n_colW, n_colH, n_den, n_dot;
-	uint32_t ldtW, ldtH;
-	size_t n_shadow;               /* uint16 elements of the four BF16 copies (0: none) */
+	size_t n_shadow;               /* uint16 elements of the two BF16 copies (0: none) */
 } job_layout;
 
 static void layout_of(uint32_t n, uint32_t m, uint32_t k, job_layout *L)
@@ -102,9 +101,7 @@ static void layout_of(uint32_t n, uint32_t m, uint32_t k, job_layout *L)
 	L->n_colW = (size_t)((n + NMFK_FIN_ROWS - 1) / NMFK_FIN_ROWS) * L->kt;
 	L->n_colH = (size_t)((m + NMFK_FIN_ROWS - 1) / NMFK_FIN_ROWS) * L->kt;
 	L->n_den = 2 * (size_t)L->kt;
-	L->ldtW = (uint32_t)round_up(n, 32);
-	L->ldtH = (uint32_t)round_up(m, 32);
-	L->n_shadow = L->kt % 32 == 0 ? 2 * ((size_t)L->ldtW + L->ldtH) * L->kt : 0;
+	L->n_shadow = L->kt % 32 == 0 ? ((size_t)n + m) * L->kt : 0;
 	L->n_dot = (size_t)nmfk_dot_kl_blocks(n, m) * 3;
 }
 
@@ -366,14 +363,10 @@ static nmfk_job *job_create(uint32_t n, uint32_t m, uint32_t k, nmfk_job *src, n
 	if (!e) e |= dalloc_job(j, (void **)&j->cls, 4 * (size_t)m, "hipMalloc(cls)");
 	if (!e) e |= dalloc_job(j, (void **)&j->last, 4 * (size_t)m, "hipMalloc(last)");
 	if (!e) e |= dalloc_job(j, (void **)&j->st, sizeof(nmfk_state), "hipMalloc(state)");
-	j->ldtW = L.ldtW, j->ldtH = L.ldtH;
 	if (!e && L.n_shadow) {
 		e |= dalloc_job(j, (void **)&j->Wb, 2 * L.n_shadow, "hipMalloc(BF16 copies)");
-		if (!e) {
-			j->Wtb = j->Wb + (size_t)n * L.kt;
-			j->Hb = j->Wtb + (size_t)L.ldtW * L.kt;
-			j->Htb = j->Hb + (size_t)m * L.kt;
-		}
+		if (!e)
+			j->Hb = j->Wb + (size_t)n * L.kt;
 	}
 	if (stream) {
 		j->stream = stream;   /* borrowed: the creator keeps it */
@@ -718,8 +711,8 @@ int nmfk_job_get_H(nmfk_job *j, float *H, size_t ldh)
 static int partial_H(nmfk_job *j, const nmfk_state *st)
 {
 	if (j->planH.path == NMFK_PATH_MFMA_BF16)
-		return nmfk_launch_partial_bf16(j->V, j->ldv, j->H, j->Wb, j->Wtb, j->ldtW, j->m, j->n, j->kt,
-						&j->planH, j->part, j->colW, j->ncolW, j->denW, st, j->stream);
+		return nmfk_launch_partial_bf16(j->V, j->ldv, j->H, j->Wb, j->m, j->n, j->kt, &j->planH, j->part,
+						j->colW, j->ncolW, j->denW, st, j->stream);
 	return nmfk_launch_partial(j->V, j->ldv, j->H, j->W, j->m, j->n, j->kt, &j->planH, j->part, j->colW,
 				   j->ncolW, j->denW, st, j->stream);
 }
@@ -727,7 +720,7 @@ static int partial_H(nmfk_job *j, const nmfk_state *st)
 static int partial_W(nmfk_job *j, const nmfk_state *st)
 {
 	if (j->planW.path == NMFK_PATH_MFMA_BF16)
-		return nmfk_launch_partial_bf16(j->Vt, j->ldvt, j->W, j->Hb, j->Htb, j->ldtH, j->n, j->m, j->kt,
+		return nmfk_launch_partial_bf16(j->Vt, j->ldvt, j->W, j->Hb, j->n, j->m, j->kt,
 						&j->planW, j->part, j->colH, j->ncolH, j->denH, st, j->stream);
 	return nmfk_launch_partial(j->Vt, j->ldvt, j->W, j->H, j->n, j->m, j->kt, &j->planW, j->part, j->colH,
 				   j->ncolH, j->denH, st, j->stream);
@@ -743,7 +736,7 @@ int nmfk_enqueue_half_H(nmfk_job *j, const nmfk_state *st)
 	const int sh = j->planW.path == NMFK_PATH_MFMA_BF16;
 	if (!e)
 		e = nmfk_launch_finish_shadow(j->part, j->planH.S, j->H, j->m, j->k, j->kt, j->denW, j->colH, 0, st,
-					      sh ? j->Hb : NULL, sh ? j->Htb : NULL, j->ldtH, j->stream);
+					      sh ? j->Hb : NULL, j->stream);
 	return e;
 }
 
@@ -753,7 +746,7 @@ int nmfk_enqueue_half_W(nmfk_job *j, const nmfk_state *st, int clamp)
 	const int sh = j->planH.path == NMFK_PATH_MFMA_BF16;
 	if (!e)
 		e = nmfk_launch_finish_shadow(j->part, j->planW.S, j->W, j->n, j->k, j->kt, j->denH, j->colW, clamp, st,
-					      sh ? j->Wb : NULL, sh ? j->Wtb : NULL, j->ldtW, j->stream);
+					      sh ? j->Wb : NULL, j->stream);
 	return e;
 }
 
@@ -761,14 +754,14 @@ int nmfk_job_shadow_for_H(nmfk_job *j)
 {
 	if (j->planH.path != NMFK_PATH_MFMA_BF16)
 		return 0;
-	return nmfk_launch_shadow(j->W, j->n, j->kt, j->Wb, j->Wtb, j->ldtW, j->stream);
+	return nmfk_launch_shadow(j->W, j->n, j->kt, j->Wb, j->stream);
 }
 
 int nmfk_job_shadow_for_W(nmfk_job *j)
 {
 	if (j->planW.path != NMFK_PATH_MFMA_BF16)
 		return 0;
-	return nmfk_launch_shadow(j->H, j->m, j->kt, j->Hb, j->Htb, j->ldtH, j->stream);
+	return nmfk_launch_shadow(j->H, j->m, j->kt, j->Hb, j->stream);
 }
 
 /* update_H(): accum_W = colsum(W) (src/NMF_routines.c:720), then the H half-step. */

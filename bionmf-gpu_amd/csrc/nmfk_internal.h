@@ -56,10 +56,9 @@ struct nmfk_job {
 	int colW_valid;
 	int vt_stale;                  /* V changed since Vt was built */
 	int owns_V;                    /* 0: V and Vt belong to another job (nmfk_job_create_shared) */
-	/* BF16 copies of W and H for the BF16 kernel path (kt % 32 == 0 jobs only): row-major
-	 * (rows x kt) and transposed (kt x ldt, ldt = rows rounded up to 32, zero padding). */
-	uint16_t *Wb, *Wtb, *Hb, *Htb;
-	uint32_t ldtW, ldtH;
+	/* BF16 copies of W and H for the BF16 kernel path (kt % 32 == 0 jobs only), row-major
+	 * (rows x kt) like the factors. */
+	uint16_t *Wb, *Hb;
 	/* Streamed jobs (nmfk_job_create_streamed): V stays in host memory and passes through
 	 * two device slots one row block at a time; V and Vt above stay NULL. */
 	int streamed;

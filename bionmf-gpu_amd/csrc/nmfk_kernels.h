@@ -108,19 +108,18 @@ int nmfk_launch_finish(const float *part, uint32_t S, float *A, uint32_t p, uint
 		       const float *den, float *colA, int clamp, const nmfk_state *st,
 		       hipStream_t stream);
 
-/* The same, also writing BF16 copies of the new A: Ab (p x kt) and Atb = A^T (kt x ldt,
- * ldt >= p); kt % 32 == 0 when Ab is given. */
+/* The same, also writing the BF16 copy of the new A: Ab (p x kt, row-major); kt % 32 == 0
+ * when Ab is given. */
 int nmfk_launch_finish_shadow(const float *part, uint32_t S, float *A, uint32_t p, uint32_t k, uint32_t kt,
-			      const float *den, float *colA, int clamp, const nmfk_state *st,
-			      uint16_t *Ab, uint16_t *Atb, uint32_t ldt, hipStream_t stream);
-/* BF16 copies of A alone (initial factors). */
-int nmfk_launch_shadow(const float *A, uint32_t p, uint32_t kt, uint16_t *Ab, uint16_t *Atb, uint32_t ldt,
-		       hipStream_t stream);
+			      const float *den, float *colA, int clamp, const nmfk_state *st, uint16_t *Ab,
+			      hipStream_t stream);
+/* The BF16 copy of A alone (initial factors). */
+int nmfk_launch_shadow(const float *A, uint32_t p, uint32_t kt, uint16_t *Ab, hipStream_t stream);
 /* Partial kernel with both contractions in BF16 (plan path NMFK_PATH_MFMA_BF16): B is read
- * from its BF16 copies Bb (q x kt) and Btb (kt x ldbt, ldbt % 32 == 0, ldbt >= q). */
-int nmfk_launch_partial_bf16(const float *U, size_t ldu, const float *A, const uint16_t *Bb, const uint16_t *Btb,
-			     uint32_t ldbt, uint32_t p, uint32_t q, uint32_t kt, const nmfk_hplan *pl, float *part,
-			     const float *colB, uint32_t ncolB, float *den, const nmfk_state *st, hipStream_t stream);
+ * from its row-major BF16 copy Bb (q x kt). */
+int nmfk_launch_partial_bf16(const float *U, size_t ldu, const float *A, const uint16_t *Bb, uint32_t p, uint32_t q,
+			     uint32_t kt, const nmfk_hplan *pl, float *part, const float *colB, uint32_t ncolB,
+			     float *den, const nmfk_state *st, hipStream_t stream);
 
 /* colA[blk][c] = sum of A[a][c] over the rows of block blk (same order as k_finish). */
 int nmfk_launch_colsum_parts(const float *A, uint32_t p, uint32_t kt, float *colA, hipStream_t stream);

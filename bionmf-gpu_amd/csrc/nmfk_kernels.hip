@@ -724,17 +724,18 @@ __global__ __launch_bounds__(64 * NA * NB, (KT == 32 ? 3 : 1)) void k_partial_mf
 //   GEMM2  acc[t][h] += R^T . B   (K = the block's 32 rows): lane (g, j)'s 8 GEMM1 outputs
 //          (both halves) are its A-operand elements, i.e. K element i of lane group g is
 //          row b = bb + 16*(i/4) + 4g + (i%4); the B operand takes B at those same rows.
-// B comes from two BF16 copies the finish kernel keeps beside the FP32 factor: row-major
-// Bb (q x KT, GEMM1: 8 consecutive columns = one 16-byte load) and transposed Btb
-// (KT x ldbt, GEMM2: rows bb+4g..+3 and bb+16+4g..+3 of one column = two 8-byte loads).
+// B comes from the row-major BF16 copy Bb (q x KT) the finish kernel keeps beside the FP32
+// factor, staged per block through LDS (see the body).
 // --------------------------------------------------------------------------------------
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 
+// LDS floats of the BF16 body: the wave groups' sums (smem_mfma), and before them each
+// wave's 32-row B slot (32 x (kt + 4) BF16).
 __host__ __device__ constexpr int smem_bf16(int kt, int na, int nb)
 {
-	return smem_mfma(kt, na, nb);
+	return smem_mfma(kt, na, nb) > na * nb * 16 * (kt + 4) ? smem_mfma(kt, na, nb) : na * nb * 16 * (kt + 4);
 }
 
 __device__ __forceinline__ bf16x8 to_bf16x8(float4 lo, float4 hi)
@@ -747,8 +748,6 @@ __device__ __forceinline__ bf16x8 to_bf16x8(float4 lo, float4 hi)
 
 struct bf16_args {
 	const __bf16 *Bb;    // q x KT, row-major
-	const __bf16 *Btb;   // KT x ldbt
-	int ldbt;
 };
 
 template <int KT, int NA, int NB>
@@ -762,8 +761,6 @@ __device__ __forceinline__ void partial_bf16_body(int bx, int by, const half_arg
 	const size_t ldu = ha.ldu;
 	const float *__restrict__ A = ha.A;
 	const __bf16 *__restrict__ Bb = bb16.Bb;
-	const __bf16 *__restrict__ Btb = bb16.Btb;
-	const int ldbt = bb16.ldbt;
 	const int p = ha.p, q = ha.q, bchunk = ha.bchunk;
 	if (bx == ha.nab) {
 		if (by == 0)
@@ -805,137 +802,129 @@ __device__ __forceinline__ void partial_bf16_body(int bx, int by, const half_arg
 	const int b_end = min(q, b_begin + bchunk);
 	const int ucol = avail > 0 ? aj : 0;
 
-	// One 32-row block: u[hb][r] = U[bb + 16hb + 4g + r][aj .. aj+3];
-	// b1[hb][ch] = Bb[bb + 16hb + j][32ch + 8g ..]; b2[h] = Btb[16h + j][rows of K group g].
-	struct blk {
-		float4 u[2][4];
-		bf16x8 b1[2][CH];
-		bf16x8 b2[H];
+	// B reaches the matrix cores through this wave's own LDS slot, one 32-row block at a time:
+	// the block's rows are fetched with whole-line 16-byte loads (B rows are KT BF16 =
+	// 2 KT bytes, so NCH chunks per row and 64 / NCH rows per load), written to the slot
+	// (row stride BS = KT + 4 elements, which spreads a column's rows over the banks), and
+	// both operand layouts are read from there: GEMM1's (row bb + 16hb + j, 8 consecutive
+	// columns) as row segments, GEMM2's (column 16h + j, rows bb + 16(i / 4) + 4g + i % 4) as
+	// a gather.  Read straight from memory, the two layouts touch 16 partly used cache lines
+	// per load, 3x the line traffic of U itself; staged, B costs whole lines and LDS reads.
+	// Each wave uses only its own slot and LDS executes a wave's accesses in order, so no
+	// barrier is needed.
+	constexpr int BS = KT + 4;
+	constexpr int NCH = KT / 8;             // 16-byte chunks per B row
+	constexpr int RPL = 64 / NCH;           // rows one load instruction covers
+	constexpr int NLD = 32 / RPL;           // loads per 32-row block
+	static_assert(32 % RPL == 0, "B staging: a block is whole loads");
+	__bf16 *__restrict__ slot = reinterpret_cast<__bf16 *>(smem) + (size_t)wave * 32 * BS;
+	const int lrow = lane / NCH, lchunk = lane % NCH;
+	uint4 braw[NLD];
+	// Rows at or past b_end (the partial last block of a split) read row b_begin instead.
+	auto fetch_b = [&](int bb, bool full) {
+#pragma unroll
+		for (int i = 0; i < NLD; i++) {
+			int b = bb + lrow + RPL * i;
+			if (!full)
+				b = b < b_end ? b : b_begin;
+			braw[i] = *reinterpret_cast<const uint4 *>(Bb + (size_t)b * KT + 8 * lchunk);
+		}
 	};
-	// Rows at or past b_end read row b_begin (in bounds) and are masked in the ratio.
-	auto load = [&](int bb, blk &x, bool full) {
+	auto stage_b = [&]() {
+#pragma unroll
+		for (int i = 0; i < NLD; i++) {
+			uint2 *d = reinterpret_cast<uint2 *>(slot + (lrow + RPL * i) * BS + 8 * lchunk);
+			d[0] = make_uint2(braw[i].x, braw[i].y);
+			d[1] = make_uint2(braw[i].z, braw[i].w);
+		}
+	};
+	bf16x8 b1[2][CH], b2[H];
+	auto read_b1 = [&]() {
+#pragma unroll
+		for (int hb = 0; hb < 2; hb++)
+#pragma unroll
+			for (int ch = 0; ch < CH; ch++) {
+				const uint2 *src = reinterpret_cast<const uint2 *>(slot + (16 * hb + j) * BS + 32 * ch + 8 * g);
+				const uint2 lo = src[0], hi = src[1];
+				const uint4 w = make_uint4(lo.x, lo.y, hi.x, hi.y);
+				__builtin_memcpy(&b1[hb][ch], &w, 16);
+			}
+	};
+	auto read_b2 = [&]() {
+#pragma unroll
+		for (int h = 0; h < H; h++)
+#pragma unroll
+			for (int i = 0; i < 8; i++)
+				b2[h][i] = slot[(16 * (i / 4) + 4 * g + i % 4) * BS + 16 * h + j];
+	};
+
+	// One block: GEMM1 S = B . A^T on both 16-row halves, the ratios U / S (FP32, hardware
+	// reciprocal, one product), GEMM2 acc += R^T . B.  MASK zeroes the ratios of rows at or
+	// past b_end.
+	auto gemm1_ratio = [&](const float4 (&u)[2][4], bf16x8 (&rt)[4], int bb, auto mask_tag) {
+		constexpr bool MASK = decltype(mask_tag)::value;
 #pragma unroll
 		for (int hb = 0; hb < 2; hb++) {
+			floatx4 sv[4];
+#pragma unroll
+			for (int t = 0; t < 4; t++) {
+				sv[t] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+				for (int ch = 0; ch < CH; ch++)
+					sv[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b1[hb][ch], aop[t][ch], sv[t], 0, 0, 0);
+			}
+#pragma unroll
+			for (int r = 0; r < 4; r++) {
+				const bool rok = !MASK || bb + 16 * hb + 4 * g + r < b_end;
+				const float uv[4] = {u[hb][r].x, u[hb][r].y, u[hb][r].z, u[hb][r].w};
+#pragma unroll
+				for (int t = 0; t < 4; t++) {
+					const float v = uv[t] * __builtin_amdgcn_rcpf(sv[t][r]);
+					rt[t][4 * hb + r] = (__bf16)(rok ? v : 0.0f);
+				}
+			}
+		}
+	};
+	auto gemm2 = [&](const bf16x8 (&rt)[4]) {
+#pragma unroll
+		for (int t = 0; t < 4; t++)
+#pragma unroll
+			for (int h = 0; h < H; h++)
+				acc[t][h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(rt[t], b2[h], acc[t][h], 0, 0, 0);
+	};
+	auto load_u = [&](int bb, float4 (&u)[2][4], bool full) {
+#pragma unroll
+		for (int hb = 0; hb < 2; hb++)
 #pragma unroll
 			for (int r = 0; r < 4; r++) {
 				int b = bb + 16 * hb + 4 * g + r;
 				if (!full)
 					b = b < b_end ? b : b_begin;
-				x.u[hb][r] = *reinterpret_cast<const float4 *>(U + (size_t)b * ldu + ucol);
+				u[hb][r] = *reinterpret_cast<const float4 *>(U + (size_t)b * ldu + ucol);
 			}
-			int b = bb + 16 * hb + j;
-			if (!full)
-				b = b < b_end ? b : b_begin;
-#pragma unroll
-			for (int ch = 0; ch < CH; ch++)
-				x.b1[hb][ch] = *reinterpret_cast<const bf16x8 *>(Bb + (size_t)b * KT + 32 * ch + 8 * g);
-		}
-		// Transposed copy: columns are padded to ldbt >= round_up(q, 32) with zeros, so a
-		// block's 32 rows are always in bounds.
-#pragma unroll
-		for (int h = 0; h < H; h++) {
-			const __bf16 *col = Btb + (size_t)(16 * h + j) * ldbt + bb + 4 * g;
-			const bf16x4 lo = *reinterpret_cast<const bf16x4 *>(col);
-			const bf16x4 hi = *reinterpret_cast<const bf16x4 *>(col + 16);
-			x.b2[h] = bf16x8{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
-		}
-	};
-	auto compute = [&](const blk &x, int bb, bool full) {
-		bf16x8 rt[4];
-#pragma unroll
-		for (int hb = 0; hb < 2; hb++) {
-			floatx4 s[4];
-#pragma unroll
-			for (int t = 0; t < 4; t++) {
-				s[t] = floatx4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-				for (int ch = 0; ch < CH; ch++)
-					s[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(x.b1[hb][ch], aop[t][ch], s[t], 0, 0, 0);
-			}
-#pragma unroll
-			for (int r = 0; r < 4; r++) {
-				const bool rok = full || bb + 16 * hb + 4 * g + r < b_end;
-				const float uv[4] = {x.u[hb][r].x, x.u[hb][r].y, x.u[hb][r].z, x.u[hb][r].w};
-#pragma unroll
-				for (int t = 0; t < 4; t++) {
-					const float v = uv[t] * __builtin_amdgcn_rcpf(s[t][r]);
-					rt[t][4 * hb + r] = (__bf16)(rok ? v : 0.0f);
-				}
-			}
-		}
-#pragma unroll
-		for (int t = 0; t < 4; t++)
-#pragma unroll
-			for (int h = 0; h < H; h++)
-				acc[t][h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(rt[t], x.b2[h], acc[t][h], 0, 0, 0);
 	};
 
 	const int nfull = (b_end - b_begin) / 32;
 	const int mine = idle ? 0 : (nfull > wb ? (nfull - wb + NB - 1) / NB : 0);
 	auto blk_row = [&](int i) { return b_begin + 32 * (wb + NB * min(i, mine - 1)); };
 	if (mine > 0) {
-		// U (the HBM stream) has a two-block ring.  The two B operands (L2-resident) are single
-		// buffers refilled as soon as their product has consumed them: GEMM1's right after
-		// GEMM1, GEMM2's right after GEMM2, so each refill overlaps the rest of the block.
-		// Refills past the end re-read this wave's last block and are never consumed.
+		// U (the HBM stream) has a two-block ring; B's next block is fetched into registers
+		// while this one is in the slot.  Refills past the end re-read this wave's last block
+		// and are never consumed.
 		float4 ur[2][2][4];
-		bf16x8 b1[2][CH], b2[H];
-		auto load_u = [&](int bb, float4 (&u)[2][4]) {
-#pragma unroll
-			for (int hb = 0; hb < 2; hb++)
-#pragma unroll
-				for (int r = 0; r < 4; r++)
-					u[hb][r] = *reinterpret_cast<const float4 *>(U + (size_t)(bb + 16 * hb + 4 * g + r) * ldu + ucol);
-		};
-		auto load_b1 = [&](int bb) {
-#pragma unroll
-			for (int hb = 0; hb < 2; hb++)
-#pragma unroll
-				for (int ch = 0; ch < CH; ch++)
-					b1[hb][ch] = *reinterpret_cast<const bf16x8 *>(Bb + (size_t)(bb + 16 * hb + j) * KT + 32 * ch + 8 * g);
-		};
-		auto load_b2 = [&](int bb) {
-#pragma unroll
-			for (int h = 0; h < H; h++) {
-				const __bf16 *col = Btb + (size_t)(16 * h + j) * ldbt + bb + 4 * g;
-				const bf16x4 lo = *reinterpret_cast<const bf16x4 *>(col);
-				const bf16x4 hi = *reinterpret_cast<const bf16x4 *>(col + 16);
-				b2[h] = bf16x8{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
-			}
-		};
 		auto step = [&](float4 (&u)[2][4], int next_b, int next_u) {
+			stage_b();
+			read_b1();
 			bf16x8 rt[4];
-#pragma unroll
-			for (int hb = 0; hb < 2; hb++) {
-				floatx4 sv[4];
-#pragma unroll
-				for (int t = 0; t < 4; t++) {
-					sv[t] = floatx4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-					for (int ch = 0; ch < CH; ch++)
-						sv[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b1[hb][ch], aop[t][ch], sv[t], 0, 0, 0);
-				}
-#pragma unroll
-				for (int r = 0; r < 4; r++) {
-					const float uv[4] = {u[hb][r].x, u[hb][r].y, u[hb][r].z, u[hb][r].w};
-#pragma unroll
-					for (int t = 0; t < 4; t++)
-						rt[t][4 * hb + r] = (__bf16)(uv[t] * __builtin_amdgcn_rcpf(sv[t][r]));
-				}
-			}
-			load_b1(next_b);
-			load_u(next_u, u);
-#pragma unroll
-			for (int t = 0; t < 4; t++)
-#pragma unroll
-				for (int h = 0; h < H; h++)
-					acc[t][h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(rt[t], b2[h], acc[t][h], 0, 0, 0);
-			load_b2(next_b);
+			gemm1_ratio(u, rt, 0, std::false_type{});
+			read_b2();
+			fetch_b(next_b, true);
+			load_u(next_u, u, true);
+			gemm2(rt);
 		};
-		load_u(blk_row(0), ur[0]);
-		load_u(blk_row(1), ur[1]);
-		load_b1(blk_row(0));
-		load_b2(blk_row(0));
+		load_u(blk_row(0), ur[0], true);
+		load_u(blk_row(1), ur[1], true);
+		fetch_b(blk_row(0), true);
 		for (int i = 0; i < mine; i += 2) {
 			step(ur[0], blk_row(i + 1), blk_row(i + 2));
 			if (i + 1 < mine)
@@ -944,12 +933,19 @@ __device__ __forceinline__ void partial_bf16_body(int bx, int by, const half_arg
 	}
 	if (!idle && (b_end - b_begin) % 32 != 0 && wb == nfull % NB) {
 		const int bb = b_begin + 32 * nfull;
-		blk x;
-		load(bb, x, false);
-		compute(x, bb, false);
+		float4 u[2][4];
+		load_u(bb, u, false);
+		fetch_b(bb, false);
+		stage_b();
+		read_b1();
+		bf16x8 rt[4];
+		gemm1_ratio(u, rt, bb, std::true_type{});
+		read_b2();
+		gemm2(rt);
 	}
 
 	if constexpr (NB > 1) {
+		__syncthreads();   // every wave is done with its B slot, which the sums below reuse
 		float (*red)[NA][4][H][4][64] = reinterpret_cast<float (*)[NA][4][H][4][64]>(smem);
 		if (wb > 0) {
 #pragma unroll
@@ -992,7 +988,7 @@ template <int KT, int NA, int NB>
 __global__ __launch_bounds__(64 * NA * NB) void k_partial_bf16(half_args h, bf16_args b, const nmfk_state *__restrict__ st,
 							     int S)
 {
-	__shared__ float smem[smem_bf16(KT, NA, NB)];
+	__shared__ __align__(16) float smem[smem_bf16(KT, NA, NB)];
 	if (st != nullptr && st->converged)
 		return;
 	int bx, by;
@@ -1046,31 +1042,30 @@ __device__ __forceinline__ void stq(float *__restrict__ p, const float (&v)[Q])
 	}
 }
 
-// BF16 copies the finish kernel keeps of the factor it updates (null: none).
+// The BF16 copy the finish kernel keeps of the factor it updates (null: none).
 struct shadow_args {
 	__bf16 *Ab;
-	__bf16 *Atb;
-	int ldt;
 };
 
-// BF16 copies of a factor A (p x KT): Ab row-major, Atb = A^T with leading dimension ldt
-// (columns p..ldt-1 stay zero).  Rows [64 bx, 64 bx + 64), 4 threads per row.
+// BF16 copy of a factor A (p x KT), row-major like A: thread (row a, quarter c0) writes its
+// KT / 4 consecutive values, 16 bytes at a time.
 template <int KT>
-__device__ __forceinline__ void store_shadow(int a, int c0, const float (&x)[KT / 4], __bf16 *__restrict__ Ab,
-					     __bf16 *__restrict__ Atb, int ldt)
+__device__ __forceinline__ void store_shadow(int a, int c0, const float (&x)[KT / 4], __bf16 *__restrict__ Ab)
 {
 	constexpr int Q = KT / 4;
+	static_assert(Q % 8 == 0, "BF16 copies: kt a multiple of 32");
 #pragma unroll
-	for (int i = 0; i < Q; i++) {
-		const __bf16 v = (__bf16)x[i];
-		Ab[(size_t)a * KT + c0 + i] = v;
-		Atb[(size_t)(c0 + i) * ldt + a] = v;
+	for (int i = 0; i < Q; i += 8) {
+		bf16x8 v;
+#pragma unroll
+		for (int e = 0; e < 8; e++)
+			v[e] = (__bf16)x[i + e];
+		*reinterpret_cast<bf16x8 *>(Ab + (size_t)a * KT + c0 + i) = v;
 	}
 }
 
 template <int KT>
-__global__ __launch_bounds__(NMFK_WG) void k_shadow(const float *__restrict__ A, int p, __bf16 *__restrict__ Ab,
-						    __bf16 *__restrict__ Atb, int ldt)
+__global__ __launch_bounds__(NMFK_WG) void k_shadow(const float *__restrict__ A, int p, __bf16 *__restrict__ Ab)
 {
 	constexpr int Q = KT / 4;
 	const int a = blockIdx.x * NMFK_FIN_ROWS + threadIdx.x / 4;
@@ -1079,7 +1074,7 @@ __global__ __launch_bounds__(NMFK_WG) void k_shadow(const float *__restrict__ A,
 		return;
 	float x[Q];
 	ldq<Q>(A + (size_t)a * KT + c0, x);
-	store_shadow<KT>(a, c0, x, Ab, Atb, ldt);
+	store_shadow<KT>(a, c0, x, Ab);
 }
 
 template <int KT>
@@ -1133,7 +1128,7 @@ __device__ __forceinline__ void finish_body(int bx, const float *__restrict__ pa
 		stq<Q>(Ar, x);
 		if constexpr (KT % 32 == 0) {
 			if (sh.Ab != nullptr)
-				store_shadow<KT>(a, c0, x, sh.Ab, sh.Atb, sh.ldt);
+				store_shadow<KT>(a, c0, x, sh.Ab);
 		}
 	} else {
 #pragma unroll
@@ -2459,12 +2454,12 @@ extern "C" int nmfk_launch_partial(const float *U, size_t ldu, const float *A, c
 
 extern "C" int nmfk_launch_finish_shadow(const float *part, uint32_t S, float *A, uint32_t p, uint32_t k,
 					 uint32_t kt, const float *den, float *colA, int clamp, const nmfk_state *st,
-					 uint16_t *Ab, uint16_t *Atb, uint32_t ldt, hipStream_t stream)
+					 uint16_t *Ab, hipStream_t stream)
 {
 	const unsigned g = (p + NMFK_FIN_ROWS - 1) / NMFK_FIN_ROWS;
-	if (Ab != nullptr && (kt % 32 != 0 || ldt < p))
+	if (Ab != nullptr && kt % 32 != 0)
 		return (int)hipErrorInvalidValue;
-	const shadow_args sh = {(__bf16 *)Ab, (__bf16 *)Atb, (int)ldt};
+	const shadow_args sh = {(__bf16 *)Ab};
 	switch (kt) {
 #define X(K)                                                                                           \
 	case K:                                                                                        \
@@ -2530,20 +2525,16 @@ extern "C" int nmfk_launch_finish(const float *part, uint32_t S, float *A, uint3
 				  const float *den, float *colA, int clamp, const nmfk_state *st,
 				  hipStream_t stream)
 {
-	return nmfk_launch_finish_shadow(part, S, A, p, k, kt, den, colA, clamp, st, nullptr, nullptr, 0, stream);
+	return nmfk_launch_finish_shadow(part, S, A, p, k, kt, den, colA, clamp, st, nullptr, stream);
 }
 
-extern "C" int nmfk_launch_shadow(const float *A, uint32_t p, uint32_t kt, uint16_t *Ab, uint16_t *Atb, uint32_t ldt,
-				  hipStream_t stream)
+extern "C" int nmfk_launch_shadow(const float *A, uint32_t p, uint32_t kt, uint16_t *Ab, hipStream_t stream)
 {
 	const unsigned g = (p + NMFK_FIN_ROWS - 1) / NMFK_FIN_ROWS;
-	if (ldt < p)
-		return (int)hipErrorInvalidValue;
 	switch (kt) {
 #define X(K)                                                                                           \
 	case K:                                                                                        \
-		hipLaunchKernelGGL((k_shadow<K>), dim3(g), dim3(NMFK_WG), 0, stream, A, (int)p, (__bf16 *)Ab,  \
-				   (__bf16 *)Atb, (int)ldt);                                             \
+		hipLaunchKernelGGL((k_shadow<K>), dim3(g), dim3(NMFK_WG), 0, stream, A, (int)p, (__bf16 *)Ab); \
 		return (int)hipGetLastError();
 		NMFK_KT_BF16_LIST(X)
 #undef X
@@ -2553,17 +2544,16 @@ extern "C" int nmfk_launch_shadow(const float *A, uint32_t p, uint32_t kt, uint1
 }
 
 template <int KT>
-static int launch_partial_bf16_t(const float *U, size_t ldu, const float *A, const uint16_t *Bb, const uint16_t *Btb,
-				 uint32_t ldbt, uint32_t p, uint32_t q, const nmfk_hplan *pl, float *part,
-				 const float *colB, uint32_t ncolB, float *den, const nmfk_state *st,
-				 hipStream_t stream)
+static int launch_partial_bf16_t(const float *U, size_t ldu, const float *A, const uint16_t *Bb, uint32_t p,
+				 uint32_t q, const nmfk_hplan *pl, float *part, const float *colB, uint32_t ncolB,
+				 float *den, const nmfk_state *st, hipStream_t stream)
 {
 	const dim3 grid = xcd_grid(pl);
 	const int na = (int)pl->ap, nb = pl->ap ? (int)(pl->nw / pl->ap) : 0;
-	if (pl->bchunk % 32 != 0 || ldbt % 32 != 0 || ldbt < q || !mfma_shape_ok(na, nb))
+	if (pl->bchunk % 32 != 0 || !mfma_shape_ok(na, nb))
 		return (int)hipErrorInvalidValue;
 	const half_args h = {U, ldu, A, nullptr, (int)p, (int)q, (int)pl->bchunk, (int)pl->nab, part, colB, (int)ncolB, den};
-	const bf16_args b = {(const __bf16 *)Bb, (const __bf16 *)Btb, (int)ldbt};
+	const bf16_args b = {(const __bf16 *)Bb};
 #define NMFK_BF16_SHAPE(NA_, NB_)                                                                        \
 	if (na == NA_ && nb == NB_) {                                                                  \
 		hipLaunchKernelGGL((k_partial_bf16<KT, NA_, NB_>), grid, dim3(64 * NA_ * NB_), 0, stream, h, b, st, (int)pl->S); \
@@ -2578,17 +2568,16 @@ static int launch_partial_bf16_t(const float *U, size_t ldu, const float *A, con
 	return (int)hipErrorInvalidValue;
 }
 
-extern "C" int nmfk_launch_partial_bf16(const float *U, size_t ldu, const float *A, const uint16_t *Bb,
-					const uint16_t *Btb, uint32_t ldbt, uint32_t p, uint32_t q, uint32_t kt,
-					const nmfk_hplan *pl, float *part, const float *colB, uint32_t ncolB,
-					float *den, const nmfk_state *st, hipStream_t stream)
+extern "C" int nmfk_launch_partial_bf16(const float *U, size_t ldu, const float *A, const uint16_t *Bb, uint32_t p,
+					uint32_t q, uint32_t kt, const nmfk_hplan *pl, float *part, const float *colB,
+					uint32_t ncolB, float *den, const nmfk_state *st, hipStream_t stream)
 {
 	if (ldu % 4 != 0 || pl->path != NMFK_PATH_MFMA_BF16)
 		return (int)hipErrorInvalidValue;
 	switch (kt) {
 #define X(K)                                                                                          \
 	case K:                                                                                       \
-		return launch_partial_bf16_t<K>(U, ldu, A, Bb, Btb, ldbt, p, q, pl, part, colB, ncolB, den, st, stream);
+		return launch_partial_bf16_t<K>(U, ldu, A, Bb, p, q, pl, part, colB, ncolB, den, st, stream);
 		NMFK_KT_BF16_LIST(X)
 #undef X
 	default:

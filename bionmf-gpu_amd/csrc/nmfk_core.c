@@ -731,8 +731,8 @@ static int partial_W(nmfk_job *j, const nmfk_state *st)
 
 /* H half: the partial kernel streams V against W (and reduces colsum(W) into denW), the
  * finish kernel applies the update to H and leaves colsum(H) partials for the W half (and
- * BF16 copies of H when the W half runs in BF16).  On the BF16 path W is read from its
- * BF16 copies, which the W half's finish (or nmfk_job_shadow_for_H) keeps current. */
+ * the BF16 copy of H when the W half runs in BF16).  On the BF16 path W is read from its
+ * BF16 copy, which the W half's finish (or nmfk_job_shadow_for_H) keeps current. */
 int nmfk_enqueue_half_H(nmfk_job *j, const nmfk_state *st)
 {
 	int e = partial_H(j, st);

@@ -112,7 +112,7 @@ int nmfk_stream_update_H(nmfk_job *j);
 int nmfk_stream_update_W(nmfk_job *j);
 void nmfk_stream_free(nmfk_job *j);
 
-/* Refreshes the BF16 copies of the factor the next half-step reads as B, if its plan is BF16. */
+/* Refreshes the BF16 copy of the factor the next half-step reads as B, if its plan is BF16. */
 int nmfk_job_shadow_for_H(nmfk_job *j);
 int nmfk_job_shadow_for_W(nmfk_job *j);
 

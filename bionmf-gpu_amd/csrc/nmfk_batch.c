@@ -23,16 +23,18 @@
 #include <string.h>
 
 /* Partial tables per half and body class: VALU half-steps without and with lane groups
- * (a factor with few rows), matrix-core jobs of kt 16 and of kt 32, each launched on its own
- * so that it runs at its own occupancy. */
-#define NCLS 4
+ * (a factor with few rows), matrix-core half-steps of kt 16 and of kt 32 in 1 x 4 waves, and
+ * the same in 2 x 2 waves (a lone job's plan over a short contraction range), each launched
+ * on its own so that it runs at its own occupancy.  Empty classes launch nothing. */
+#define NCLS 6
 enum { TAB_PART_H0, TAB_PART_W0 = TAB_PART_H0 + NCLS, TAB_FIN_H = TAB_PART_W0 + NCLS, TAB_FIN_W, TAB_TEST, NTAB };
 
 static int half_cls(const nmfk_job *j, int half)
 {
+	const nmfk_hplan *p = half ? &j->planW : &j->planH;
 	if (j->kt <= 8)
-		return (half ? j->planW.lb : j->planH.lb) > 1 ? 1 : 0;
-	return j->kt == 16 ? 2 : 3;
+		return p->lb > 1 ? 1 : 0;
+	return (j->kt == 16 ? 2 : 3) + (p->ap == 2 ? 2 : 0);
 }
 
 struct nmfk_batch {
@@ -172,12 +174,13 @@ uint32_t nmfk_batch_active(const nmfk_batch *b)
 	return b->active;
 }
 
-/* Plans the batched bodies run: VALU <KT, 4, 4> for KT 4/8, MFMA <KT, 1, 4> for KT 16/32. */
+/* Plans the batched bodies run: VALU <KT, 4, 4> for KT 4/8, MFMA <KT, 1, 4> and <KT, 2, 2>
+ * for KT 16/32. */
 static int plan_batchable(const nmfk_hplan *p, uint32_t kt)
 {
 	if (kt <= 8)
 		return p->path == NMFK_PATH_VALU && p->ap == 4 && p->nw == 4;
-	return (kt == 16 || kt == 32) && p->path == NMFK_PATH_MFMA32 && p->ap == 1 && p->nw == 4;
+	return (kt == 16 || kt == 32) && p->path == NMFK_PATH_MFMA32 && (p->ap == 1 || p->ap == 2) && p->nw == 4;
 }
 
 static void fill_desc(nmfk_bjob *d, const nmfk_job *j)

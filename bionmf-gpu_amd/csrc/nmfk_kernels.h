@@ -170,8 +170,8 @@ typedef struct nmfk_tile {
 
 /* half: 0 = H half (U = V, A = H, B = W), 1 = W half (U = Vt, A = W, B = H).  cls: the body
  * class of every tile's half-step: 0 = vector ALUs (kt 4/8) without lane groups, 1 = vector
- * ALUs with lane groups (plan lb > 1), 2 and 3 = matrix cores at kt 16 and 32; tiles of other
- * classes are skipped. */
+ * ALUs with lane groups (plan lb > 1), 2 and 3 = matrix cores at kt 16 and 32 in 1 x 4 waves,
+ * 4 and 5 = the same in 2 x 2 waves (plan ap 2); tiles of other classes are skipped. */
 int nmfk_launch_batch_partial(const nmfk_bjob *jobs, const nmfk_tile *tiles, uint32_t ntiles, int half,
 			      int cls, uint32_t grid, hipStream_t stream);
 int nmfk_launch_batch_finish(const nmfk_bjob *jobs, const nmfk_tile *tiles, uint32_t ntiles, int half,

@@ -1905,18 +1905,29 @@ __device__ __forceinline__ void batch_valu_body(int bx, int by, const half_args 
 	}
 }
 
-// Four batched partial kernels, one per body class, so each runs at its own occupancy: the
+// Six batched partial kernels, one per body class, so each runs at its own occupancy: the
 // VALU bodies (KT 4/8) of half-steps without lane groups, those with lane groups (few rows
-// of the updated factor: their variants need more registers), the matrix-core body of KT 16
-// and that of KT 32; a batch launches each over its own tile table.
-template <int CLS>
-constexpr int smem_batch_partial()
+// of the updated factor: their variants need more registers), the matrix-core bodies of KT 16
+// and of KT 32 in 1 x 4 waves, and the same in 2 x 2 waves (the plan of a lone job over a
+// short contraction range); a batch launches each over its own tile table.
+__host__ __device__ constexpr int batch_cls_kt(int cls)
 {
-	return CLS <= 1 ? smem_valu(8, 4, 4) : smem_mfma(CLS == 2 ? 16 : 32, 1, 4);
+	return (cls == 2 || cls == 4) ? 16 : 32;
+}
+__host__ __device__ constexpr int batch_cls_na(int cls)
+{
+	return cls >= 4 ? 2 : 1;
 }
 
 template <int CLS>
-__global__ __launch_bounds__(NMFK_WG, (CLS == 0 || CLS == 2 ? 4 : 1)) void k_batch_partial(const nmfk_bjob *__restrict__ jobs,
+constexpr int smem_batch_partial()
+{
+	return CLS <= 1 ? smem_valu(8, 4, 4)
+			: smem_mfma(batch_cls_kt(CLS), batch_cls_na(CLS), 4 / batch_cls_na(CLS));
+}
+
+template <int CLS>
+__global__ __launch_bounds__(NMFK_WG, (CLS == 0 || CLS == 2 || CLS == 4 ? 4 : 1)) void k_batch_partial(const nmfk_bjob *__restrict__ jobs,
 							   const nmfk_tile *__restrict__ tiles, int ntiles, int half)
 {
 	__shared__ float smem[smem_batch_partial<CLS>()];
@@ -1924,8 +1935,10 @@ __global__ __launch_bounds__(NMFK_WG, (CLS == 0 || CLS == 2 ? 4 : 1)) void k_bat
 		const nmfk_tile tl = tiles[t];
 		const nmfk_bjob &J = jobs[tl.job];
 		// a half-step of another class would not fit this kernel's body: never run it here
-		const uint32_t lb = half == 0 ? J.planH.lb : J.planW.lb;
-		const bool fits = CLS <= 1 ? (J.kt <= 8 && (lb > 1) == (CLS == 1)) : J.kt == (CLS == 2 ? 16u : 32u);
+		const nmfk_hplan &pl = half == 0 ? J.planH : J.planW;
+		const uint32_t lb = pl.lb;
+		const bool fits = CLS <= 1 ? (J.kt <= 8 && (lb > 1) == (CLS == 1))
+					   : (J.kt == (uint32_t)batch_cls_kt(CLS) && pl.ap == (uint32_t)batch_cls_na(CLS));
 		if (!fits || !J.live || J.st->converged)
 			continue;
 		const half_args h = batch_half(J, half);
@@ -1940,7 +1953,7 @@ __global__ __launch_bounds__(NMFK_WG, (CLS == 0 || CLS == 2 ? 4 : 1)) void k_bat
 			else
 				batch_valu_body<8>(tl.bx, tl.by, h, smem);
 		} else {
-			partial_mfma_body<CLS == 2 ? 16 : 32, 1, 4>(tl.bx, tl.by, h, smem);
+			partial_mfma_body<batch_cls_kt(CLS), batch_cls_na(CLS), 4 / batch_cls_na(CLS)>(tl.bx, tl.by, h, smem);
 		}
 		__syncthreads();   // the next tile reuses the LDS
 	}
@@ -2316,6 +2329,15 @@ extern "C" void nmfk_plan_half_target(uint32_t p, uint32_t q, uint32_t kt, int p
 	// were changed.  At 50000 x 2000, k = 64 the W half takes 125.5 us against 132.7 us.
 	if (path == NMFK_PATH_MFMA_BF16 && na == 1 && nb == 4 && q < 4096u)
 		na = 2, nb = 2;
+	// FP32 at kt 16 and 32 over 1000-4095 rows of U (the W half of V 20000 x 1000): 2 x 2
+	// waves with at least 12 steps per wave group, for a job planned on its own (no workgroup
+	// target) with the shape and step tunables at their defaults.  At 20000 x 1000, k = 16 the
+	// W partial takes 18.5-18.7 us in 3 splits against 20.6-21.8 us in 2; shorter ranges (600
+	// rows) lose 1-2 us.
+	int min_steps = g_tune[NMFK_TUNE_MIN_STEPS];
+	if (path == NMFK_PATH_MFMA32 && kt <= 32u && target_wgs == 0 && na == 1 && nb == 4 && min_steps == 8 &&
+	    q >= 1000u && q < 4096u)
+		na = 2, nb = 2, min_steps = 12;
 	const bool valu = path == NMFK_PATH_VALU;
 	const uint32_t ap = valu ? default_ap(kt) : (uint32_t)na;
 	const uint32_t nw = valu ? 4u : (uint32_t)(na * nb);
@@ -2333,7 +2355,7 @@ extern "C" void nmfk_plan_half_target(uint32_t p, uint32_t q, uint32_t kt, int p
 			   : (uint32_t)g_tune[valu ? NMFK_TUNE_WGS_VALU
 						   : path == NMFK_PATH_MFMA_BF16 ? NMFK_TUNE_WGS_BF16 : NMFK_TUNE_WGS_MFMA];
 	uint32_t S = (target + nab - 1u) / nab;
-	const uint32_t min_rows = steppers * rows_per_step * (uint32_t)g_tune[NMFK_TUNE_MIN_STEPS];
+	const uint32_t min_rows = steppers * rows_per_step * (uint32_t)min_steps;
 	const uint32_t smax = (q + min_rows - 1u) / min_rows;
 	if (S > smax)
 		S = smax;
@@ -2651,6 +2673,12 @@ extern "C" int nmfk_launch_batch_partial(const nmfk_bjob *jobs, const nmfk_tile 
 		break;
 	case 3:
 		hipLaunchKernelGGL((k_batch_partial<3>), g, dim3(NMFK_WG), 0, stream, jobs, tiles, (int)ntiles, half);
+		break;
+	case 4:
+		hipLaunchKernelGGL((k_batch_partial<4>), g, dim3(NMFK_WG), 0, stream, jobs, tiles, (int)ntiles, half);
+		break;
+	case 5:
+		hipLaunchKernelGGL((k_batch_partial<5>), g, dim3(NMFK_WG), 0, stream, jobs, tiles, (int)ntiles, half);
 		break;
 	default:
 		return (int)hipErrorInvalidValue;

@@ -19,6 +19,9 @@ def lib(pkg):
     lib = pkg.load()
     lib.nmfk_plan_half_path.restype = None
     lib.nmfk_plan_half_path.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, C.POINTER(HPlan)]
+    lib.nmfk_plan_half_target.restype = None
+    lib.nmfk_plan_half_target.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, C.c_uint32,
+                                          C.POINTER(HPlan)]
     saved = {key: lib.nmfk_tune_get(key) for key in DEFAULTS}
     yield lib
     for key, value in saved.items():
@@ -70,8 +73,10 @@ def test_plan_covers_both_ranges(lib, p, q, kt, path):
         assert pl.lb == 8 or p > 64 * pl.ap // (2 * pl.lb)
     else:
         step, rows_a = (32 if path == PATH_BF16 else 16), 64 * pl.ap
-        # matrix cores: 1 x 4 waves, or 2 x 2 for BF16 over fewer than 4096 rows of U
-        shape = (2, 4) if path == PATH_BF16 and q < 4096 else (1, 4)
+        # matrix cores: 1 x 4 waves, or 2 x 2 for BF16 over fewer than 4096 rows of U and for
+        # FP32 at kt 16/32 over 1000-4095 rows
+        short = (path == PATH_BF16 and q < 4096) or (path == PATH_MFMA32 and kt <= 32 and 1000 <= q < 4096)
+        shape = (2, 4) if short else (1, 4)
         assert (pl.ap, pl.nw) == shape and pl.lb == 1
     assert pl.nab == -(-p // rows_a)
     assert pl.bchunk % step == 0 and pl.S >= 1 and pl.S <= DEFAULTS[3]
@@ -108,6 +113,25 @@ def test_tunables_move_the_plan(lib):
     assert lib.nmfk_tune_set(4, 3) == 0                     # 3 x 2 is no kernel shape: default 1 x 4
     pl = plan(lib, 1000, 20000, 16)
     assert (pl.ap, pl.nw) == (1, 4)
+
+
+def test_fp32_short_range_plan(lib):
+    # c1's W half (20000 rows of W against the 1000 rows of Vt): 2 x 2 waves, 128 rows of W per
+    # workgroup (157 columns), at least 12 steps of 16 rows per wave group, so at most
+    # ceil(1000 / 384) = 3 splits of 336 rows.
+    w = plan(lib, 20000, 1000, 16)
+    assert (w.path, w.ap, w.nw, w.nab, w.S, w.bchunk) == (PATH_MFMA32, 2, 4, 157, 3, 336)
+    assert (plan(lib, 20000, 1000, 32).ap, plan(lib, 20000, 999, 16).ap, plan(lib, 20000, 4096, 16).ap) == (2, 1, 1)
+    assert plan(lib, 20000, 2000, 64, PATH_MFMA32).ap == 1          # kt 64 keeps 1 x 4 in FP32
+    # an explicit workgroup target (batched plans) keeps 1 x 4
+    t = HPlan()
+    lib.nmfk_plan_half_target(20000, 1000, 16, PATH_AUTO, 64, C.byref(t))
+    assert (t.ap, t.nw) == (1, 4)
+    # so do changed step or shape tunables
+    assert lib.nmfk_tune_set(2, 6) == 0
+    assert plan(lib, 20000, 1000, 16).ap == 1
+    assert lib.nmfk_tune_set(2, 8) == 0 and lib.nmfk_tune_set(4, 4) == 0 and lib.nmfk_tune_set(5, 1) == 0
+    assert (plan(lib, 20000, 1000, 16).ap, plan(lib, 20000, 1000, 16).nw) == (4, 4)
 
 
 @pytest.mark.parametrize("n,m,kt,ok", [(20000, 1000, 16, True), (20000, 1024, 16, True), (20000, 1025, 16, False),

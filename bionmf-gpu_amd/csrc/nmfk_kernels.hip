@@ -1934,11 +1934,16 @@ __global__ __launch_bounds__(NMFK_WG, (CLS == 0 || CLS == 2 || CLS == 4 ? 4 : 1)
 	for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
 		const nmfk_tile tl = tiles[t];
 		const nmfk_bjob &J = jobs[tl.job];
-		// a half-step of another class would not fit this kernel's body: never run it here
-		const nmfk_hplan &pl = half == 0 ? J.planH : J.planW;
-		const uint32_t lb = pl.lb;
-		const bool fits = CLS <= 1 ? (J.kt <= 8 && (lb > 1) == (CLS == 1))
-					   : (J.kt == (uint32_t)batch_cls_kt(CLS) && pl.ap == (uint32_t)batch_cls_na(CLS));
+		// a half-step of another class would not fit this kernel's body: never run it here.  The
+		// 1 x 4 classes leave the shape to the tables (the check would cost the kt-16 body a
+		// spill at its 128-register bound).
+		bool fits;
+		if constexpr (CLS <= 1)
+			fits = J.kt <= 8 && ((half == 0 ? J.planH.lb : J.planW.lb) > 1) == (CLS == 1);
+		else if constexpr (CLS <= 3)
+			fits = J.kt == (uint32_t)batch_cls_kt(CLS);
+		else
+			fits = J.kt == (uint32_t)batch_cls_kt(CLS) && (half == 0 ? J.planH.ap : J.planW.ap) == 2u;
 		if (!fits || !J.live || J.st->converged)
 			continue;
 		const half_args h = batch_half(J, half);
@@ -2304,6 +2309,42 @@ extern "C" int nmfk_tune_get(int key)
 	return (key >= 0 && key < NMFK_TUNE_COUNT) ? g_tune[key] : -1;
 }
 
+// Rows of U per split for S splits of q rows, whole wave steps.
+static uint32_t split_rows(uint32_t q, uint32_t S, uint32_t rows_per_step)
+{
+	const uint32_t bc = (q + S - 1u) / S;
+	return (bc + rows_per_step - 1u) / rows_per_step * rows_per_step;
+}
+
+// Split count of a lone FP32 matrix-core job.  These launches are bound by their matrix work
+// per SIMD, so a launch lasts about as long as its busiest CU needs: the workgroups that CU
+// runs times the rows each streams (q / S).  The XCD-aware order gives each of the 8 XCDs
+// ceil(tiles / 8) workgroups over its 32 CUs, of which at most R four-wave workgroups are
+// resident per CU (4 at kt 16, 3 at kt 32, by their register use); beyond that they queue
+// and even out.  Among S0 / 2 .. S0 splits this takes the least such load, the fewest splits
+// on a tie (a shorter finish), keeping at least 240 workgroups (about one per CU).  At
+// 20000 x 1000, k = 16 the H half's 40 splits become 30: partial 20.7-21.5 -> 18.6-19.1 us.
+static uint32_t balanced_splits(uint32_t q, uint32_t nab, uint32_t S0, uint32_t rows_per_step, uint32_t kt)
+{
+	const uint32_t R = kt <= 16u ? 4u : 3u;
+	auto load = [&](uint32_t s) {
+		const uint32_t per = ((nab + 1u) * s + 7u) / 8u;
+		const double w = per <= 32u * R ? (double)((per + 31u) / 32u) : per / 32.0;
+		return w / s;
+	};
+	uint32_t best = S0;
+	double bl = load(S0);
+	for (uint32_t s = S0; s >= 1u && 2u * s >= S0; s--) {
+		const uint32_t se = (q + split_rows(q, s, rows_per_step) - 1u) / split_rows(q, s, rows_per_step);
+		if (se != S0 && (nab + 1u) * se < 240u)
+			continue;
+		const double l = load(se);
+		if (l < bl * (1.0 - 1e-9) || (l <= bl * (1.0 + 1e-9) && se < best))
+			best = se, bl = l;
+	}
+	return best;
+}
+
 extern "C" void nmfk_plan_half_path(uint32_t p, uint32_t q, uint32_t kt, int path, nmfk_hplan *pl)
 {
 	nmfk_plan_half_target(p, q, kt, path, 0, pl);
@@ -2329,50 +2370,64 @@ extern "C" void nmfk_plan_half_target(uint32_t p, uint32_t q, uint32_t kt, int p
 	// were changed.  At 50000 x 2000, k = 64 the W half takes 125.5 us against 132.7 us.
 	if (path == NMFK_PATH_MFMA_BF16 && na == 1 && nb == 4 && q < 4096u)
 		na = 2, nb = 2;
-	// FP32 at kt 16 and 32 over 1000-4095 rows of U (the W half of V 20000 x 1000): 2 x 2
-	// waves with at least 12 steps per wave group, for a job planned on its own (no workgroup
-	// target) with the shape and step tunables at their defaults.  At 20000 x 1000, k = 16 the
-	// W partial takes 18.5-18.7 us in 3 splits against 20.6-21.8 us in 2; shorter ranges (600
-	// rows) lose 1-2 us.
-	int min_steps = g_tune[NMFK_TUNE_MIN_STEPS];
-	if (path == NMFK_PATH_MFMA32 && kt <= 32u && target_wgs == 0 && na == 1 && nb == 4 && min_steps == 8 &&
-	    q >= 1000u && q < 4096u)
-		na = 2, nb = 2, min_steps = 12;
+	// Two rules for an FP32 job at kt 16 or 32 planned on its own (no workgroup target, as
+	// batched replans give) with the matrix-core tunables at their defaults.  First, over
+	// 1000-4095 rows of U (the W half of V 20000 x 1000): 2 x 2 waves with at least 12 steps
+	// per wave group, where that still gives 240 workgroups or more.  At 20000 x 1000, k = 16
+	// the W partial takes 18.5-18.7 us in 3 splits against 20.6-21.8 us in 2; shorter ranges
+	// (600 rows) lose 1-2 us, and so do jobs too small to fill the GPU either way (V 2500 x
+	// 1000: 63 workgroups against 85).  Second, the split count is balanced over the CUs
+	// (balanced_splits).
+	const bool lone_fp32 = path == NMFK_PATH_MFMA32 && kt <= 32u && target_wgs == 0 && na == 1 && nb == 4 &&
+			       g_tune[NMFK_TUNE_MIN_STEPS] == 8 && g_tune[NMFK_TUNE_WGS_MFMA] == 1024 &&
+			       g_tune[NMFK_TUNE_MAX_SPLITS] == 64;
 	const bool valu = path == NMFK_PATH_VALU;
-	const uint32_t ap = valu ? default_ap(kt) : (uint32_t)na;
-	const uint32_t nw = valu ? 4u : (uint32_t)(na * nb);
-	// VALU lane groups: as many as leave every lane a row of A (p <= 64 ap / lb)
-	uint32_t lb = 1;
-	if (valu)
-		while (lb < 8 && p <= 64u * ap / (2u * lb))
-			lb *= 2;
-	const uint32_t rows_per_step = valu ? lb : path == NMFK_PATH_MFMA_BF16 ? 32u : 16u;   // rows of U per wave step
-	const uint32_t steppers = valu ? nw : (uint32_t)nb;        // wave groups splitting the rows
-	const uint32_t rows_per_wg = valu ? 64u / lb * ap : 64u * ap;   // rows a per workgroup
-	const uint32_t nab = (p + rows_per_wg - 1u) / rows_per_wg;
-	const uint32_t target =
-		target_wgs ? target_wgs
-			   : (uint32_t)g_tune[valu ? NMFK_TUNE_WGS_VALU
-						   : path == NMFK_PATH_MFMA_BF16 ? NMFK_TUNE_WGS_BF16 : NMFK_TUNE_WGS_MFMA];
-	uint32_t S = (target + nab - 1u) / nab;
-	const uint32_t min_rows = steppers * rows_per_step * (uint32_t)min_steps;
-	const uint32_t smax = (q + min_rows - 1u) / min_rows;
-	if (S > smax)
-		S = smax;
-	if (S > (uint32_t)g_tune[NMFK_TUNE_MAX_SPLITS])
-		S = (uint32_t)g_tune[NMFK_TUNE_MAX_SPLITS];
-	if (S < 1)
-		S = 1;
-	uint32_t bchunk = (q + S - 1u) / S;
-	bchunk = (bchunk + rows_per_step - 1u) / rows_per_step * rows_per_step;
-	S = (q + bchunk - 1u) / bchunk;
-	pl->nab = nab;
-	pl->S = S;
-	pl->bchunk = bchunk;
-	pl->ap = ap;
-	pl->nw = nw;
-	pl->path = (uint32_t)path;
-	pl->lb = lb;
+	auto fill = [&](int na_, int nb_, int min_steps) {
+		const uint32_t ap = valu ? default_ap(kt) : (uint32_t)na_;
+		const uint32_t nw = valu ? 4u : (uint32_t)(na_ * nb_);
+		// VALU lane groups: as many as leave every lane a row of A (p <= 64 ap / lb)
+		uint32_t lb = 1;
+		if (valu)
+			while (lb < 8 && p <= 64u * ap / (2u * lb))
+				lb *= 2;
+		const uint32_t rows_per_step = valu ? lb : path == NMFK_PATH_MFMA_BF16 ? 32u : 16u;   // rows of U per wave step
+		const uint32_t steppers = valu ? nw : (uint32_t)nb_;       // wave groups splitting the rows
+		const uint32_t rows_per_wg = valu ? 64u / lb * ap : 64u * ap;   // rows a per workgroup
+		const uint32_t nab = (p + rows_per_wg - 1u) / rows_per_wg;
+		const uint32_t target =
+			target_wgs ? target_wgs
+				   : (uint32_t)g_tune[valu ? NMFK_TUNE_WGS_VALU
+							   : path == NMFK_PATH_MFMA_BF16 ? NMFK_TUNE_WGS_BF16 : NMFK_TUNE_WGS_MFMA];
+		uint32_t S = (target + nab - 1u) / nab;
+		const uint32_t min_rows = steppers * rows_per_step * (uint32_t)min_steps;
+		const uint32_t smax = (q + min_rows - 1u) / min_rows;
+		if (S > smax)
+			S = smax;
+		if (S > (uint32_t)g_tune[NMFK_TUNE_MAX_SPLITS])
+			S = (uint32_t)g_tune[NMFK_TUNE_MAX_SPLITS];
+		if (S < 1)
+			S = 1;
+		uint32_t bchunk = split_rows(q, S, rows_per_step);
+		S = (q + bchunk - 1u) / bchunk;
+		if (lone_fp32) {
+			S = balanced_splits(q, nab, S, rows_per_step, kt);
+			bchunk = split_rows(q, S, rows_per_step);
+			S = (q + bchunk - 1u) / bchunk;
+		}
+		pl->nab = nab;
+		pl->S = S;
+		pl->bchunk = bchunk;
+		pl->ap = ap;
+		pl->nw = nw;
+		pl->path = (uint32_t)path;
+		pl->lb = lb;
+	};
+	if (lone_fp32 && q >= 1000u && q < 4096u) {
+		fill(2, 2, 12);
+		if ((pl->nab + 1u) * pl->S >= 240u)
+			return;
+	}
+	fill(na, nb, g_tune[NMFK_TUNE_MIN_STEPS]);
 }
 
 extern "C" void nmfk_plan_half(uint32_t p, uint32_t q, uint32_t kt, nmfk_hplan *pl)

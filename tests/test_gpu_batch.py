@@ -15,16 +15,18 @@ from conftest import rel_err
 pytestmark = pytest.mark.gpu
 
 # (n, m, k, seed_v, seed_init, nIters, stop_threshold): k covers every batched width
-# (kt 4, 8, 16, 32), and the H halves of the 2000 x 300 and 1500 x 120 jobs (1000-4095 rows
-# of V at kt 32 and 16) plan 2 x 2 matrix-core waves; the last two cannot converge in time
-# and run the leftover nIters % T iterations after their test budget.
+# (kt 4, 8, 16, 32), and the 4000 x 3000 (kt 16, both halves) and 4200 x 2800 (kt 32, W half)
+# jobs plan 2 x 2 matrix-core waves (1000-4095 rows of U, at least 240 workgroups); the last
+# two cannot converge in time and run the leftover nIters % T iterations after their test
+# budget.
 MIX = [
     (700, 150, 3, 1, 11, 400, 4),
     (1200, 90, 6, 2, 12, 400, 4),
     (333, 200, 10, 3, 13, 400, 4),
     (900, 64, 17, 4, 14, 400, 4),
     (2000, 300, 32, 5, 15, 400, 4),
-    (1500, 120, 14, 9, 19, 400, 4),
+    (4000, 3000, 16, 9, 19, 400, 4),
+    (4200, 2800, 24, 10, 20, 400, 4),
     (64, 20, 2, 6, 16, 400, 4),
     (500, 77, 5, 7, 17, 123, 1000),
     (256, 300, 16, 8, 18, 57, 1000),
@@ -52,8 +54,9 @@ def assert_same(a, b, ra, rb, what):
 
 def test_batch_equals_jobs_alone(nmfk, orc):
     pairs = [twins(nmfk, orc, s) for s in MIX]
-    # 2 x 2 waves own 128 rows of H per workgroup (1 x 4: 64)
-    assert [pairs[i][0].plan()["H"]["nab"] for i in (4, 5)] == [3, 1]
+    # 2 x 2 waves own 128 rows of the updated factor per workgroup (1 x 4: 64)
+    plans = [pairs[i][0].plan() for i in (5, 6)]
+    assert [plans[0]["H"]["nab"], plans[0]["W"]["nab"], plans[1]["W"]["nab"]] == [24, 32, 33]
     batch = nmfk.Batch(T, 16)
     for (a, _), s in zip(pairs, MIX):
         batch.add(a, s[5], s[6])

@@ -74,10 +74,14 @@ def test_plan_covers_both_ranges(lib, p, q, kt, path):
     else:
         step, rows_a = (32 if path == PATH_BF16 else 16), 64 * pl.ap
         # matrix cores: 1 x 4 waves, or 2 x 2 for BF16 over fewer than 4096 rows of U and for
-        # FP32 at kt 16/32 over 1000-4095 rows
-        short = (path == PATH_BF16 and q < 4096) or (path == PATH_MFMA32 and kt <= 32 and 1000 <= q < 4096)
-        shape = (2, 4) if short else (1, 4)
-        assert (pl.ap, pl.nw) == shape and pl.lb == 1
+        # FP32 at kt 16/32 over 1000-4095 rows where that gives at least 240 workgroups
+        assert pl.nw == 4 and pl.lb == 1
+        if path == PATH_BF16 and q < 4096:
+            assert pl.ap == 2
+        elif path == PATH_MFMA32 and kt <= 32 and 1000 <= q < 4096:
+            assert pl.ap == 1 or (pl.nab + 1) * pl.S >= 240
+        else:
+            assert pl.ap == 1
     assert pl.nab == -(-p // rows_a)
     assert pl.bchunk % step == 0 and pl.S >= 1 and pl.S <= DEFAULTS[3]
     assert (pl.S - 1) * pl.bchunk < q <= pl.S * pl.bchunk   # every split holds rows, together all q
@@ -96,11 +100,13 @@ def test_bf16_plans_of_the_dense_rank_workload(lib):
 
 def test_tunables_move_the_plan(lib):
     # c1's H half, 16 workgroup columns.  The minimum of 8 steps per wave caps the splits at
-    # ceil(20000 / (4 * 16 * 8)) = 40 under the 1024-workgroup target.  512 workgroups ask
-    # for 32 splits: 625 rows, rounded up to 640 (16-row steps), 32 splits.  2048 workgroups
-    # with 1 step per wave ask for 128, capped at 64: 313 rows -> 320, 63 splits.
+    # ceil(20000 / (4 * 16 * 8)) = 40 under the 1024-workgroup target; balanced over the CUs
+    # that becomes 30 (672 rows each).  With any tunable changed the planner does what the
+    # tunables say: 512 workgroups ask for 32 splits: 625 rows, rounded up to 640 (16-row
+    # steps), 32 splits.  2048 workgroups with 1 step per wave ask for 128, capped at 64: 313
+    # rows -> 320, 63 splits.
     base = plan(lib, 1000, 20000, 16)
-    assert (base.nab, base.S, base.bchunk) == (16, 40, 512)
+    assert (base.nab, base.S, base.bchunk) == (16, 30, 672)
     assert lib.nmfk_tune_set(1, DEFAULTS[1] // 2) == 0
     assert (plan(lib, 1000, 20000, 16).S, plan(lib, 1000, 20000, 16).bchunk) == (32, 640)
     assert lib.nmfk_tune_set(2, 1) == 0 and lib.nmfk_tune_set(1, 2 * DEFAULTS[1]) == 0
@@ -115,6 +121,32 @@ def test_tunables_move_the_plan(lib):
     assert (pl.ap, pl.nw) == (1, 4)
 
 
+def busiest_cu_load(nab, S, kt):
+    # workgroups the busiest CU of an XCD runs (8 XCDs of 32 CUs, 4 resident four-wave
+    # workgroups per CU at kt 16 and 3 at kt 32, queued ones evening out) per split
+    per = -(-(nab + 1) * S // 8)
+    resident = 4 if kt <= 16 else 3
+    return (-(-per // 32) if per <= 32 * resident else per / 32) / S
+
+
+@pytest.mark.parametrize("p,q,kt,S", [(1000, 20000, 16, 30), (1000, 20000, 32, 30), (1000, 10000, 16, 15),
+                                      (1000, 30000, 16, 30), (500, 20000, 16, 28), (500, 60000, 16, 56),
+                                      (1000, 40000, 16, 63), (2000, 20000, 16, 32), (3000, 8000, 32, 16),
+                                      (30000, 1000, 16, 2), (8000, 3000, 32, 4), (14000, 600, 16, 2)])
+def test_balanced_splits(lib, p, q, kt, S):
+    # A lone FP32 job's split count: the least busiest-CU load among half the default split
+    # count up to the default, the fewest splits on a tie, at least 240 workgroups.
+    pl = plan(lib, p, q, kt)
+    assert pl.S == S
+    t = HPlan()
+    lib.nmfk_plan_half_target(p, q, kt, PATH_AUTO, 1024, C.byref(t))     # the same job, unbalanced
+    assert (t.nab, t.ap) == (pl.nab, pl.ap) or pl.ap == 2
+    if pl.ap == 1:
+        assert S <= t.S <= 2 * S + 1
+        assert busiest_cu_load(pl.nab, S, kt) <= busiest_cu_load(t.nab, t.S, kt) + 1e-12
+    assert (pl.nab + 1) * S >= 240 or S == t.S
+
+
 def test_fp32_short_range_plan(lib):
     # c1's W half (20000 rows of W against the 1000 rows of Vt): 2 x 2 waves, 128 rows of W per
     # workgroup (157 columns), at least 12 steps of 16 rows per wave group, so at most
@@ -122,6 +154,8 @@ def test_fp32_short_range_plan(lib):
     w = plan(lib, 20000, 1000, 16)
     assert (w.path, w.ap, w.nw, w.nab, w.S, w.bchunk) == (PATH_MFMA32, 2, 4, 157, 3, 336)
     assert (plan(lib, 20000, 1000, 32).ap, plan(lib, 20000, 999, 16).ap, plan(lib, 20000, 4096, 16).ap) == (2, 1, 1)
+    # too few workgroups in 2 x 2 (V 2500 x 1000: 9 x 7 = 63 tiles for the H half): 1 x 4
+    assert (plan(lib, 1000, 2500, 16).ap, plan(lib, 2500, 1000, 16).ap) == (1, 1)
     assert plan(lib, 20000, 2000, 64, PATH_MFMA32).ap == 1          # kt 64 keeps 1 x 4 in FP32
     # an explicit workgroup target (batched plans) keeps 1 x 4
     t = HPlan()

@@ -89,17 +89,24 @@ static void layout_of(uint32_t n, uint32_t m, uint32_t k, job_layout *L)
 	L->n_Vt = (size_t)m * L->ldvt;
 	L->n_W = (size_t)n * L->kt;
 	L->n_H = (size_t)m * L->kt;
-	/* partial buffer: large enough for every kernel path a job may be switched to */
+	/* partial buffer: large enough for every kernel path a job may be switched to, planned on
+	 * its own or replanned to the path's default workgroup target (which skips the lone-job
+	 * rules, and may split more) */
 	L->n_part = 0;
 	for (int path = NMFK_PATH_VALU; path <= NMFK_PATH_MFMA_BF16; path++) {
-		nmfk_hplan h, w;
-		nmfk_plan_half_path(m, n, L->kt, path, &h);
-		nmfk_plan_half_path(n, m, L->kt, path, &w);
-		size_t pH = (size_t)h.S * m * L->kt, pW = (size_t)w.S * n * L->kt;
-		if (pH > L->n_part)
-			L->n_part = pH;
-		if (pW > L->n_part)
-			L->n_part = pW;
+		const int key = path == NMFK_PATH_VALU ? NMFK_TUNE_WGS_VALU
+				: path == NMFK_PATH_MFMA_BF16 ? NMFK_TUNE_WGS_BF16 : NMFK_TUNE_WGS_MFMA;
+		for (int t = 0; t < 2; t++) {
+			const uint32_t target = t ? (uint32_t)nmfk_tune_get(key) : 0u;
+			nmfk_hplan h, w;
+			nmfk_plan_half_target(m, n, L->kt, path, target, &h);
+			nmfk_plan_half_target(n, m, L->kt, path, target, &w);
+			size_t pH = (size_t)h.S * m * L->kt, pW = (size_t)w.S * n * L->kt;
+			if (pH > L->n_part)
+				L->n_part = pH;
+			if (pW > L->n_part)
+				L->n_part = pW;
+		}
 	}
 	L->n_colW = (size_t)((n + NMFK_FIN_ROWS - 1) / NMFK_FIN_ROWS) * L->kt;
 	L->n_colH = (size_t)((m + NMFK_FIN_ROWS - 1) / NMFK_FIN_ROWS) * L->kt;

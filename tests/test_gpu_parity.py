@@ -309,6 +309,33 @@ def test_transpose_beyond_65535_row_tiles(nmfk, orc):
 
 # ---------------------------------------------------------------- full-size runs against FP64
 
+@pytest.mark.parametrize("n,m,k", [(20000, 1000, 16), (60000, 500, 16), (8000, 3000, 32)])
+def test_replan_to_targets(nmfk, n, m, k):
+    # A job's split buffer covers its lone plan and a replan to the default workgroup target,
+    # which skips the lone-job rules (2 x 2 waves, balanced splits) and may split more: every
+    # target runs, target 0 gives the lone plan back, and the half-steps of the two plans
+    # agree within the FP32 bar (their sums run in different orders).
+    jobs = [nmfk.Job(n, m, k) for _ in range(2)]
+    try:
+        for j in jobs:
+            j.gen_V_planted(7, 8, 0.3)
+            j.init_random(3)
+        lone = jobs[0].plan()
+        for t in (8, 64, 1024, 0):
+            jobs[1].replan(t)
+        assert jobs[1].plan() == lone
+        jobs[1].replan(1024)
+        assert jobs[1].plan()["H"]["S"] >= lone["H"]["S"]
+        for j in jobs:
+            j.update_H()
+            j.update_W()
+        assert rel_err(jobs[1].get_H(), jobs[0].get_H()) < 1e-5
+        assert rel_err(jobs[1].get_W(), jobs[0].get_W()) < 1e-5
+    finally:
+        for j in jobs:
+            j.close()
+
+
 def _large_fixture(name):
     f = GOLDEN / f"{name}.npz"
     if not f.exists():

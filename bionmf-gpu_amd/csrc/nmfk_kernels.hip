@@ -2335,7 +2335,8 @@ static uint32_t balanced_splits(uint32_t q, uint32_t nab, uint32_t S0, uint32_t 
 	uint32_t best = S0;
 	double bl = load(S0);
 	for (uint32_t s = S0; s >= 1u && 2u * s >= S0; s--) {
-		const uint32_t se = (q + split_rows(q, s, rows_per_step) - 1u) / split_rows(q, s, rows_per_step);
+		const uint32_t rows = split_rows(q, s, rows_per_step);
+		const uint32_t se = (q + rows - 1u) / rows;   // splits that whole wave steps give
 		if (se != S0 && (nab + 1u) * se < 240u)
 			continue;
 		const double l = load(se);

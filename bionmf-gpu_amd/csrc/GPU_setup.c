@@ -157,7 +157,10 @@ int setup_GPU(size_t mem_size, bool do_classf)
 	NpP = N, MpP = M, bN = 0, bM = 0;
 	Mp = (index_t)j->ldv;
 	MpPp = Mp;
+	/* The job's padded rank (which depends on N and M, unknown at initialize_GPU) is the one
+	 * the host matrices W and H take; the caller sizes them after this call. */
 	Kp = j->kt;
+	init_kernel_params(Kp);
 	/* Streamed: the caller's first upload of block_N.BL[0] rows lands in slot 0 (the job
 	 * copies every block itself); resident: one copy of V, as in the reference when V fits. */
 	d_Vrow = streamed ? j->Vb[0] : j->V;

@@ -77,7 +77,7 @@ typedef struct job_layout {
 
 static void layout_of(uint32_t n, uint32_t m, uint32_t k, job_layout *L)
 {
-	L->kt = nmfk_kt(k);
+	L->kt = nmfk_job_kt(n, m, k);
 	L->ldv = round_up(m, 4);
 	/* Vt rows start on 256-byte boundaries, so that each wave's 64-column strip of a row is
 	 * two whole cache lines (at 50000 x 2000, k = 64, the BF16 W partial takes 88.5 instead of

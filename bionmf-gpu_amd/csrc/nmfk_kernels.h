@@ -5,7 +5,7 @@
  * and returns 0 or a hipError_t value.  Matrices are row-major FP32:
  *   V  n x ldv   (ldv >= m, ldv % 4 == 0)        Vt  m x ldvt (ldvt >= n, ldvt % 4 == 0)
  *   W  n x kt    H  m x kt   (H stored transposed, as the reference stores it)
- * kt is the padded rank (nmfk_kt(k)); columns k..kt-1 of W and H are kept at zero.
+ * kt is the job's padded rank (nmfk_job_kt(n, m, k)); columns k..kt-1 of W and H stay zero.
  */
 #ifndef NMFK_KERNELS_H
 #define NMFK_KERNELS_H
@@ -33,8 +33,11 @@ typedef struct nmfk_state {
 /* Largest rank handled by the kernels. */
 #define NMFK_KMAX 64
 
-/* Padded rank used for storage and kernel selection. */
+/* Padded rank of a rank alone (4 or 8 up to k = 8, then 16, 32, 64). */
 uint32_t nmfk_kt(uint32_t k);
+/* Padded rank a job of n x m at rank k uses for storage and kernel selection: nmfk_kt(k),
+ * except that ranks up to 8 pad to 16 (FP32 matrix cores) unless n or m is below 32. */
+uint32_t nmfk_job_kt(uint32_t n, uint32_t m, uint32_t k);
 
 /* Rows of A whose column sums one partial row covers (k_finish / k_colsum_parts). */
 #define NMFK_FIN_ROWS 64

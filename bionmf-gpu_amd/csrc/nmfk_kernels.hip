@@ -2267,14 +2267,28 @@ static inline unsigned grid1d(size_t total, unsigned block)
 
 extern "C" uint32_t nmfk_kt(uint32_t k)
 {
-	// k <= 8 stays on the VALU kernel (memory-bound there); 9..16 use one 16-column group
-	// of the FP32 MFMA kernel; larger ranks are padded to the 32-column chunks of the BF16
-	// MFMA kernel (32 or 64), so every rank above 32 can run in BF16.
+	// k <= 8: 4 or 8 columns (the VALU kernel's widths); 9..16 use one 16-column group of
+	// the FP32 MFMA kernel; larger ranks are padded to the 32-column chunks of the BF16 MFMA
+	// kernel (32 or 64), so every rank above 32 can run in BF16.
 	if (k <= 8)
 		return (k + 3u) & ~3u;
 	if (k <= 16)
 		return 16u;
 	return (k + 31u) & ~31u;
+}
+
+// A job's padded rank.  Ranks up to 8 are padded to 16 for the FP32 MFMA kernel, unless V
+// has fewer than 32 rows or columns: there the VALU kernel's lane groups (several rows of U
+// per wave) fit the narrow factor and keep its 4 or 8 columns.  The MFMA kernel does 2-4x the
+// padded flops but runs far more waves per SIMD than the VALU kernel's plans allow.  Per
+// iteration, VALU kt 4/8 against MFMA kt 16: 20000 x 1000, k = 8: 96.1 against 46.8 us; k = 3:
+// 75.8 / 46.3; 10000 x 500, k = 8: 57.6 / 28.5; 5000 x 200, k = 6: 37.1 / 21.8; 1000 x 60000,
+// k = 5: 265 / 131; 5000 x 38, k = 3: 20.3 / 19.1; but 60000 x 20, k = 4: 27.8 / 33.3.
+extern "C" uint32_t nmfk_job_kt(uint32_t n, uint32_t m, uint32_t k)
+{
+	if (k <= 8 && n >= 32u && m >= 32u)
+		return 16u;
+	return nmfk_kt(k);
 }
 
 static inline uint32_t default_ap(uint32_t kt)

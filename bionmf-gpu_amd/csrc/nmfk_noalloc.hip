@@ -99,7 +99,7 @@ extern "C" int nmfk_dot_kl_noalloc(const float *V, size_t ldv, const float *W, c
 	if (rc)
 		return rc;
 	hipStream_t s = static_cast<hipStream_t>(stream);
-	int e = nmfk_launch_dot_kl(V, ldv, W, H, n, m, k, nmfk_kt(k), parts, s);
+	int e = nmfk_launch_dot_kl(V, ldv, W, H, n, m, k, nmfk_job_kt(n, m, k), parts, s);
 	if (e == 0) {
 		hipLaunchKernelGGL(k_sum_parts_in_order, dim3(1), dim3(64), 0, s, parts, nblocks, out);
 		e = (int)hipGetLastError();

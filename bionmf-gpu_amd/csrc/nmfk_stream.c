@@ -92,7 +92,7 @@ typedef struct slayout {
 
 static void stream_layout(uint32_t n, uint32_t m, uint32_t k, uint32_t block_rows, slayout *L)
 {
-	L->kt = nmfk_kt(k);
+	L->kt = nmfk_job_kt(n, m, k);
 	L->ldv = rup(m, 4);
 	uint32_t br = block_rows;
 	if (br == 0) {

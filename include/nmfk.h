@@ -104,10 +104,10 @@ int nmfk_job_dot_kl(nmfk_job *job, double *dot_V, double *dot_VWH, double *kl);
 /* nmfk_job_dot_kl on caller-owned device buffers, allocating nothing.  No counterpart in
  * the reference: it is for callers that cannot allocate device memory (code under HIP
  * graph capture, tight serving loops, harnesses that forbid hipMalloc).  V is n x ldv
- * (ldv >= m), W n x kt and H m x kt with kt = nmfk_kt(k) and columns k..kt-1 zero.  Writes
- * {<V,V>, <V-WH,V-WH>, D(V||WH)} as 3 doubles to device memory <out>, asynchronously on
- * <stream> (a hipStream_t; NULL = the null stream): the same bits nmfk_job_dot_kl returns
- * for the same buffers.  The FP64 partial sums go to a static device array of
+ * (ldv >= m), W n x kt and H m x kt with kt = nmfk_job_kt(n, m, k) and columns k..kt-1
+ * zero.  Writes {<V,V>, <V-WH,V-WH>, D(V||WH)} as 3 doubles to device memory <out>,
+ * asynchronously on <stream> (a hipStream_t; NULL = the null stream): the same bits
+ * nmfk_job_dot_kl returns for the same buffers.  The FP64 partial sums go to a static device array of
  * nmfk_dot_kl_noalloc_max_bytes() bytes (1 MiB) per device, held until the process ends;
  * 24 bytes per 256 columns x 64 row splits, so m <= 174592; larger m is refused.  Calls
  * that overlap on one device share that array and race: serialise them. */
@@ -118,19 +118,26 @@ size_t nmfk_dot_kl_noalloc_max_bytes(void);
 /* Synchronizes the job's stream. */
 int nmfk_job_sync(nmfk_job *job);
 
-/* Kernel path of the half-step contractions: 0 = automatic (vector ALUs for k <= 8, FP32
- * matrix cores for k 9..32, BF16 matrix cores above), 1 = vector ALUs, 2 = FP32 matrix
- * cores (k padded to 16), 3 = BF16 matrix cores (k padded to 32; FP32 accumulation). */
+/* Kernel path of the half-step contractions: 0 = automatic (vector ALUs for k <= 8 when V
+ * has fewer than 32 rows or columns, FP32 matrix cores up to k = 32, BF16 matrix cores
+ * above), 1 = vector ALUs, 2 = FP32 matrix cores (k padded to 16), 3 = BF16 matrix cores (k
+ * padded to 32; FP32 accumulation).  The job's padded rank (nmfk_job_kt) is fixed when it is
+ * created, and a path its padding does not fit falls back to the next simpler one. */
 int nmfk_job_set_kernel_path(nmfk_job *job, int path);
+/* Padded rank of a job of n x m at rank k (the row pitch of W and H): 4 or 8 for k <= 8
+ * when n or m is below 32 (the vector-ALU kernel's widths), else 16 up to k = 16, then 32
+ * or 64.  Padding small ranks to 16 puts them on the FP32 matrix cores, which run them
+ * 1.1-2x faster than the vector ALUs at every width but the narrowest. */
+uint32_t nmfk_job_kt(uint32_t n, uint32_t m, uint32_t k);
 int nmfk_job_kernel_path(const nmfk_job *job);
 
 /* Iteration schedule of nmfk_job_run.  NMFK_SCHED_TWO_PASS: one pass over V per half-step
  * (V for the H half, its transposed copy for the W half), the schedule batches and the
  * drop-in half-steps use.  NMFK_SCHED_ONE_PASS: one pass over V per iteration, each row
  * block giving its new W rows and then its share of the next H half-step's sums (FP32
- * matrix cores; padded rank 16, i.e. k 9..16, and m <= 1024 on the default kernel path); it
- * reads V half as often but does the same matrix-core work, on which both are bound at these
- * ranks.  NMFK_SCHED_AUTO (default): two passes.  Both keep the reference's update order,
+ * matrix cores; padded rank 16, i.e. k <= 16 on V of 32 rows and columns or more, and
+ * m <= 1024 on the default kernel path); it reads V half as often but does the same
+ * matrix-core work, on which both are bound at these ranks.  NMFK_SCHED_AUTO (default): two passes.  Both keep the reference's update order,
  * clamp and stop rule; their sums are added in different orders, so they agree to rounding,
  * not bit for bit. */
 #define NMFK_SCHED_AUTO 0

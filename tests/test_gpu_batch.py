@@ -15,13 +15,15 @@ from conftest import rel_err
 pytestmark = pytest.mark.gpu
 
 # (n, m, k, seed_v, seed_init, nIters, stop_threshold): k covers every batched width
-# (kt 4, 8, 16, 32), and the 4000 x 3000 (kt 16, both halves) and 4200 x 2800 (kt 32, W half)
-# jobs plan 2 x 2 matrix-core waves (1000-4095 rows of U, at least 240 workgroups); the last
-# two cannot converge in time and run the leftover nIters % T iterations after their test
-# budget.
+# (kt 4 and 8 on V of fewer than 32 columns, where small ranks stay on the vector ALUs, with
+# and without lane groups; kt 16 and 32), and the 4000 x 3000 (kt 16, both halves) and
+# 4200 x 2800 (kt 32, W half) jobs plan 2 x 2 matrix-core waves (1000-4095 rows of U, at
+# least 240 workgroups); the last two cannot converge in time and run the leftover
+# nIters % T iterations after their test budget.
 MIX = [
     (700, 150, 3, 1, 11, 400, 4),
     (1200, 90, 6, 2, 12, 400, 4),
+    (3000, 24, 6, 11, 21, 400, 4),
     (333, 200, 10, 3, 13, 400, 4),
     (900, 64, 17, 4, 14, 400, 4),
     (2000, 300, 32, 5, 15, 400, 4),
@@ -55,7 +57,7 @@ def assert_same(a, b, ra, rb, what):
 def test_batch_equals_jobs_alone(nmfk, orc):
     pairs = [twins(nmfk, orc, s) for s in MIX]
     # 2 x 2 waves own 128 rows of the updated factor per workgroup (1 x 4: 64)
-    plans = [pairs[i][0].plan() for i in (5, 6)]
+    plans = [pairs[i][0].plan() for i in (6, 7)]
     assert [plans[0]["H"]["nab"], plans[0]["W"]["nab"], plans[1]["W"]["nab"]] == [24, 32, 33]
     batch = nmfk.Batch(T, 16)
     for (a, _), s in zip(pairs, MIX):

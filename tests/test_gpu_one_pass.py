@@ -122,9 +122,11 @@ def test_one_pass_is_deterministic(nmfk, orc):
 
 def test_schedule_selection(nmfk, orc):
     # Automatic is two passes everywhere; one pass is available (and refused elsewhere) for
-    # the FP32 matrix-core path at padded rank 16 with m <= 1024.
+    # the FP32 matrix-core path at padded rank 16 with m <= 1024 (k <= 8 pads to 16 unless V
+    # has fewer than 32 rows or columns).
     for (n, m, k), ok in [((500, 300, 16), True), ((500, 1024, 9), True), ((500, 1025, 16), False),
-                          ((500, 300, 8), False), ((500, 300, 17), False), ((500, 300, 40), False)]:
+                          ((500, 300, 8), True), ((500, 24, 8), False), ((500, 300, 17), False),
+                          ((500, 300, 40), False)]:
         with nmfk.Job(n, m, k) as j:
             assert j.schedule() == TWO, (n, m, k)
             if ok:

@@ -52,9 +52,9 @@ def half_tol(job):
 @pytest.mark.parametrize("path", [0, 1, 2, 3])
 @pytest.mark.parametrize("k", KS)
 def test_half_steps_match_oracle(nmfk, orc, k, path):
-    # path 0: automatic (vector ALUs for k <= 8, FP32 matrix cores for 9..32, BF16 matrix
-    # cores above), 1: vector ALUs, 2: FP32 matrix cores, 3: BF16 matrix cores (each falls
-    # back to the next simpler path where the padded rank does not fit it).
+    # path 0: automatic (at 700 x 150 FP32 matrix cores up to k = 32, ranks up to 8 padded to
+    # 16; BF16 matrix cores above), 1: vector ALUs, 2: FP32 matrix cores, 3: BF16 matrix cores
+    # (each falls back to the next simpler path where the padded rank does not fit it).
     n, m = 700, 150
     job, V = make_job(nmfk, orc, n, m, k, seed_v=k, seed_init=k + 100)
     with job:
@@ -206,10 +206,12 @@ def test_stop_rule_on_device_matches_oracle_trace(nmfk, orc):
         assert nz >= 3 and len(set(ref[ref > 0].tolist())) >= 2   # several distinct non-zero counts
 
 
-@pytest.mark.parametrize("n,m,k,seed", [(3000, 97, 3, 41), (1200, 300, 7, 42), (2500, 150, 12, 43)])
+@pytest.mark.parametrize("n,m,k,seed", [(3000, 97, 3, 41), (1200, 300, 7, 42), (2500, 150, 12, 43),
+                                        (3000, 30, 5, 44)])
 def test_device_difference_counts_equal_oracle(nmfk, orc, n, m, k, seed):
-    # Larger m (more pairs, up to 44850) and every kernel path (VALU at k <= 8, matrix cores
-    # at k = 12): the device's per-test differences equal the FP64 oracle's pairwise counts.
+    # Larger m (more pairs, up to 44850) and both kernel families (matrix cores at padded rank
+    # 16, the vector ALUs at kt 8 on the 30-column V): the device's per-test differences equal
+    # the FP64 oracle's pairwise counts.
     V = orc.gen_lognormal(n, m, seed, 5.0, 1.5)
     W0, H0 = orc.init_WH(n, m, k, seed + 100)
     o = orc.nmf(V, W0, H0, 400, 10, 40, fp64=True, max_trace=100)

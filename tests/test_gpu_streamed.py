@@ -17,8 +17,8 @@ from conftest import GOLDEN, rel_err
 pytestmark = pytest.mark.gpu
 
 # (n, m, k, block_rows): several blocks with a ragged last one, exactly divisible, one
-# block larger than n, 64-row blocks; k over the vector-ALU (kt 4, 8) and the matrix-core
-# (kt 16, 32) kernel paths.
+# block larger than n, 64-row blocks; k over the matrix-core (kt 16, 32; ranks up to 8 pad to
+# 16) and, on V of fewer than 32 columns, the vector-ALU (kt 4, 8) kernel paths.
 CASES = [
     (3000, 200, 3, 512),
     (3000, 200, 16, 1024),
@@ -26,6 +26,8 @@ CASES = [
     (777, 150, 5, 64),
     (1000, 90, 10, 4096),
     (4096, 128, 8, 1024),
+    (3000, 24, 3, 512),
+    (2000, 28, 7, 640),
 ]
 
 

@@ -196,6 +196,7 @@ def load(path: os.PathLike | str | None = None):
     _sig(lib, "nmfk_job_kernel_path", i32, [vp])
     _sig(lib, "nmfk_tune_set", i32, [i32, i32])
     _sig(lib, "nmfk_tune_get", i32, [i32])
+    _sig(lib, "nmfk_job_kt", u32, [u32, u32, u32])
     _sig(lib, "nmfk_measure_stream", i32, [sz, u32, u32, C.POINTER(dbl), C.POINTER(dbl)])
     _sig(lib, "nmfk_job_create_shared", vp, [vp, u32])
     _sig(lib, "nmfk_job_replan", i32, [vp, u32])
@@ -267,6 +268,11 @@ def tune_set(key: int, value: int):
 
 def tune_get(key: int) -> int:
     return load().nmfk_tune_get(key)
+
+
+def job_kt(n: int, m: int, k: int) -> int:
+    """Padded rank (row pitch of W and H) a job of n x m at rank k uses (nmfk_job_kt)."""
+    return int(load().nmfk_job_kt(n, m, k))
 
 
 def measure_stream(nbytes: int = 1 << 30, iters: int = 10, blocks: int = 4096):

@@ -168,6 +168,16 @@ def test_fp32_short_range_plan(lib):
     assert (plan(lib, 20000, 1000, 16).ap, plan(lib, 20000, 1000, 16).nw) == (4, 4)
 
 
+@pytest.mark.parametrize("n,m,k,kt", [(20000, 1000, 3, 16), (5000, 38, 3, 16), (5000, 200, 8, 16), (32, 32, 1, 16),
+                                     (60000, 20, 4, 4), (31, 5000, 8, 8), (3000, 24, 6, 8), (64, 20, 2, 4),
+                                     (100, 100, 9, 16), (100, 100, 16, 16), (100, 100, 17, 32), (100, 100, 33, 64),
+                                     (5, 3, 2, 4)])
+def test_job_padded_rank(pkg, n, m, k, kt):
+    # Ranks up to 8 pad to 16 (FP32 matrix cores) unless V has fewer than 32 rows or columns,
+    # where the vector ALUs keep 4 or 8 columns; larger ranks pad to 16, 32 or 64.
+    assert pkg.job_kt(n, m, k) == kt
+
+
 @pytest.mark.parametrize("n,m,kt,ok", [(20000, 1000, 16, True), (20000, 1024, 16, True), (20000, 1025, 16, False),
                                        (20000, 1000, 32, False), (20000, 1000, 8, False), (15, 64, 16, True),
                                        (1, 1, 16, True), (50000, 2000, 64, False), (0, 10, 16, False)])

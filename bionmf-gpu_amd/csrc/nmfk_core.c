@@ -85,6 +85,8 @@ static void layout_of(uint32_t n, uint32_t m, uint32_t k, job_layout *L)
 	L->ldvt = round_up(n, 64);
 	nmfk_plan_half(m, n, L->kt, &L->planH);
 	nmfk_plan_half(n, m, L->kt, &L->planW);
+	nmfk_plan_set_rank(&L->planH, L->kt, k);
+	nmfk_plan_set_rank(&L->planW, L->kt, k);
 	L->n_V = (size_t)n * L->ldv;
 	L->n_Vt = (size_t)m * L->ldvt;
 	L->n_W = (size_t)n * L->kt;
@@ -457,6 +459,8 @@ int nmfk_job_set_kernel_path(nmfk_job *j, int path)
 	nmfk_hplan h, w;
 	nmfk_plan_half_path(j->m, j->n, j->kt, path, &h);
 	nmfk_plan_half_path(j->n, j->m, j->kt, path, &w);
+	nmfk_plan_set_rank(&h, j->kt, j->k);
+	nmfk_plan_set_rank(&w, j->kt, j->k);
 	if ((size_t)h.S * j->m * j->kt > j->part_cap || (size_t)w.S * j->n * j->kt > j->part_cap) {
 		nmfk_set_error("nmfk_job_set_kernel_path: plan needs more partial space than the job holds "
 			       "(tunables changed after nmfk_job_create)");
@@ -478,6 +482,8 @@ int nmfk_job_replan(nmfk_job *j, uint32_t target_wgs)
 	nmfk_hplan h, w;
 	nmfk_plan_half_target(j->m, j->n, j->kt, (int)j->planH.path, target_wgs, &h);
 	nmfk_plan_half_target(j->n, j->m, j->kt, (int)j->planW.path, target_wgs, &w);
+	nmfk_plan_set_rank(&h, j->kt, j->k);
+	nmfk_plan_set_rank(&w, j->kt, j->k);
 	if ((size_t)h.S * j->m * j->kt > j->part_cap || (size_t)w.S * j->n * j->kt > j->part_cap) {
 		nmfk_set_error("nmfk_job_replan: plan needs more partial space than the job holds");
 		return NMFK_EINVAL;

@@ -57,6 +57,8 @@ typedef struct nmfk_hplan {
 	uint32_t nw;      /* waves per workgroup */
 	uint32_t path;    /* NMFK_PATH_* actually used */
 	uint32_t lb;      /* VALU: lane groups of a wave taking different rows of U (1, 2, 4, 8) */
+	uint32_t ke;      /* FP32 MFMA at kt 16: columns of the rank GEMM1 sums (4, 8 or 16; the
+			     planner gives kt, nmfk_plan_set_rank narrows it to the job's rank) */
 } nmfk_hplan;
 
 /* Launch-shape tunables of the planner (process-wide; jobs planned afterwards use them). */
@@ -74,6 +76,8 @@ int nmfk_tune_get(int key);
 
 /* Chooses the partial-kernel grid for updating a p x kt factor against q rows of U. */
 void nmfk_plan_half(uint32_t p, uint32_t q, uint32_t kt, nmfk_hplan *pl);
+/* Narrows a plan's GEMM1 width to rank k (FP32 matrix cores at kt 16 with k <= 8). */
+void nmfk_plan_set_rank(nmfk_hplan *pl, uint32_t kt, uint32_t k);
 void nmfk_plan_half_path(uint32_t p, uint32_t q, uint32_t kt, int path, nmfk_hplan *pl);
 /* Same, aiming for <target> workgroups instead of the process tunable (0 = tunable). */
 void nmfk_plan_half_target(uint32_t p, uint32_t q, uint32_t kt, int path, uint32_t target, nmfk_hplan *pl);

@@ -423,6 +423,7 @@ __global__ __launch_bounds__(64 * NW) void k_partial(half_args h, const nmfk_sta
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef int intx4 __attribute__((ext_vector_type(4)));
+typedef int intx2 __attribute__((ext_vector_type(2)));
 
 #define NMFK_RSRC_FLAGS 0x00020000   // buffer resource word 3 of the gfx9 family
 
@@ -438,19 +439,55 @@ __device__ __forceinline__ float4 bload4(__amdgpu_buffer_rsrc_t r, int voff, int
 	return make_float4(__int_as_float(v.x), __int_as_float(v.y), __int_as_float(v.z), __int_as_float(v.w));
 }
 
+__device__ __forceinline__ float2 bload2(__amdgpu_buffer_rsrc_t r, int voff, int soff)
+{
+	const intx2 v = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
+	return make_float2(__int_as_float(v.x), __int_as_float(v.y));
+}
+
 __device__ __forceinline__ float bload1(__amdgpu_buffer_rsrc_t r, int voff, int soff)
 {
 	return __int_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
 }
 
+// QN consecutive floats (1, 2 or 4) into the first QN entries of a float4.
+template <int QN>
+__device__ __forceinline__ float4 bloadq(__amdgpu_buffer_rsrc_t r, int voff)
+{
+	if constexpr (QN == 4)
+		return bload4(r, voff, 0);
+	else if constexpr (QN == 2) {
+		const float2 v = bload2(r, voff, 0);
+		return make_float4(v.x, v.y, 0.f, 0.f);
+	} else
+		return make_float4(bload1(r, voff, 0), 0.f, 0.f, 0.f);
+}
+template <int QN>
+__device__ __forceinline__ float4 loadq(const float *p)
+{
+	if constexpr (QN == 4)
+		return *reinterpret_cast<const float4 *>(p);
+	else if constexpr (QN == 2) {
+		const float2 v = *reinterpret_cast<const float2 *>(p);
+		return make_float4(v.x, v.y, 0.f, 0.f);
+	} else
+		return make_float4(p[0], 0.f, 0.f, 0.f);
+}
+
 // Workgroup = NA x NB waves: NA waves side by side in a (lockstep over the same rows of U),
 // NB groups of them taking the split's 16-row blocks round-robin; the NB partial sums of
 // each row a are added through LDS in a fixed order.
-template <int KT, int NA, int NB>
+// KE: columns of the rank GEMM1 sums (KT, or 4 / 8 at KT 16 for a rank up to 4 / 8, whose
+// other columns are zero).  Lane group g supplies QN = KE / 4 consecutive columns per 16-column
+// group, column 16h + QN g + qq for K-chunk qq < QN, so a rank up to 8 takes two GEMM1 chunks
+// instead of four and a rank up to 4 one.
+template <int KT, int NA, int NB, int KE = KT>
 __device__ __forceinline__ void partial_mfma_body(int bx, int by, const half_args &ha, float *__restrict__ smem)
 {
 	static_assert(KT % 16 == 0, "MFMA path needs KT % 16 == 0");
+	static_assert(KE == KT || (KT == 16 && (KE == 4 || KE == 8)), "GEMM1 width: KT, or 4 / 8 at KT 16");
 	constexpr int H = KT / 16;     // 16-column groups of the rank
+	constexpr int QN = KE >= 16 ? 4 : KE / 4;
 	const float *__restrict__ U = ha.U;
 	const size_t ldu = ha.ldu;
 	const float *__restrict__ A = ha.A;
@@ -472,15 +509,15 @@ __device__ __forceinline__ void partial_mfma_body(int bx, int by, const half_arg
 	const int aj = a0 + 4 * j;               // first of this lane's 4 rows a (one per tile)
 	const int avail = p - aj;                // tiles t < avail are real rows
 
-	// GEMM1 B operand: lane (g, j) supplies A[aj + t][16h + 4g + qq] for K-chunk (h, qq).
+	// GEMM1 B operand: lane (g, j) supplies A[aj + t][16h + QN g + qq] for K-chunk (h, qq).
 	floatx4 aop[4][H];
 #pragma unroll
 	for (int t = 0; t < 4; t++) {
 		const bool ok = t < avail;
-		const float *Ar = A + (size_t)(ok ? aj + t : 0) * KT + 4 * g;
+		const float *Ar = A + (size_t)(ok ? aj + t : 0) * KT + QN * g;
 #pragma unroll
 		for (int h = 0; h < H; h++) {
-			const float4 v = *reinterpret_cast<const float4 *>(Ar + 16 * h);
+			const float4 v = loadq<QN>(Ar + 16 * h);
 			aop[t][h] = ok ? floatx4{v.x, v.y, v.z, v.w} : floatx4{0.f, 0.f, 0.f, 0.f};
 		}
 	}
@@ -509,11 +546,11 @@ __device__ __forceinline__ void partial_mfma_body(int bx, int by, const half_arg
 	}
 #pragma unroll
 	for (int h = 0; h < H; h++)
-		b1off[h] = (j * KT + 16 * h + 4 * g) * 4;
+		b1off[h] = (j * KT + 16 * h + QN * g) * 4;
 
 	// Everything one 16-row block needs:
 	//   u[r]      U[bb + 4g + r][aj .. aj+3]
-	//   b1[h]     GEMM1 A operand, B[bb + j][16h + 4g + qq]
+	//   b1[h]     GEMM1 A operand, B[bb + j][16h + QN g + qq], qq < QN
 	//   b2[r][h]  GEMM2 B operand, B[bb + 4g + r][16h + j]
 	struct blk {
 		float4 u[4];
@@ -532,7 +569,7 @@ __device__ __forceinline__ void partial_mfma_body(int bx, int by, const half_arg
 		}
 #pragma unroll
 		for (int h = 0; h < H; h++)
-			x.b1[h] = bload4(rB, b1off[h], 0);
+			x.b1[h] = bloadq<QN>(rB, b1off[h]);
 	};
 	// Partial last block of the matrix: rows at or past b_end read row b_begin instead.
 	auto load_tail = [&](int bb, blk &x) {
@@ -547,17 +584,17 @@ __device__ __forceinline__ void partial_mfma_body(int bx, int by, const half_arg
 				x.b2[r][h] = Br[16 * h];
 		}
 		const int b = bb + j;
-		const float *Br = B + (size_t)(b < b_end ? b : b_begin) * KT + 4 * g;
+		const float *Br = B + (size_t)(b < b_end ? b : b_begin) * KT + QN * g;
 #pragma unroll
 		for (int h = 0; h < H; h++)
-			x.b1[h] = *reinterpret_cast<const float4 *>(Br + 16 * h);
+			x.b1[h] = loadq<QN>(Br + 16 * h);
 	};
 	// The three stages of one block.  MASK zeroes rows at or past b_end.
 	auto gemm1 = [&](const blk &x, floatx4 (&s)[4], bool b1ok) {
 #pragma unroll
 		for (int h = 0; h < H; h++)
 #pragma unroll
-			for (int qq = 0; qq < 4; qq++) {
+			for (int qq = 0; qq < QN; qq++) {
 				const float4 b1 = x.b1[h];
 				const float bq = qq == 0 ? b1.x : qq == 1 ? b1.y : qq == 2 ? b1.z : b1.w;
 				const float bv = b1ok ? bq : 0.0f;
@@ -699,7 +736,7 @@ __device__ __forceinline__ void partial_mfma_body(int bx, int by, const half_arg
 		}
 }
 
-template <int KT, int NA, int NB>
+template <int KT, int NA, int NB, int KE = KT>
 __global__ __launch_bounds__(64 * NA * NB, (KT == 32 ? 3 : 1)) void k_partial_mfma(half_args h, const nmfk_state *__restrict__ st,
 										       int S)
 {
@@ -709,7 +746,7 @@ __global__ __launch_bounds__(64 * NA * NB, (KT == 32 ? 3 : 1)) void k_partial_mf
 	int bx, by;
 	if (!xcd_tile(h.nab + 1, (h.nab + 1) * S, bx, by))
 		return;
-	partial_mfma_body<KT, NA, NB>(bx, by, h, smem);
+	partial_mfma_body<KT, NA, NB, KE>(bx, by, h, smem);
 }
 
 // --------------------------------------------------------------------------------------
@@ -1958,7 +1995,19 @@ __global__ __launch_bounds__(NMFK_WG, (CLS == 0 || CLS == 2 || CLS == 4 ? 4 : 1)
 			else
 				batch_valu_body<8>(tl.bx, tl.by, h, smem);
 		} else {
-			partial_mfma_body<batch_cls_kt(CLS), batch_cls_na(CLS), 4 / batch_cls_na(CLS)>(tl.bx, tl.by, h, smem);
+			constexpr int KT = batch_cls_kt(CLS), NA = batch_cls_na(CLS), NB = 4 / NA;
+			if constexpr (KT == 16) {
+				// a rank up to 4 or 8 sums only its columns in GEMM1 (nmfk_plan_set_rank)
+				const uint32_t ke = half == 0 ? J.planH.ke : J.planW.ke;
+				if (ke == 4)
+					partial_mfma_body<16, NA, NB, 4>(tl.bx, tl.by, h, smem);
+				else if (ke == 8)
+					partial_mfma_body<16, NA, NB, 8>(tl.bx, tl.by, h, smem);
+				else
+					partial_mfma_body<16, NA, NB>(tl.bx, tl.by, h, smem);
+			} else {
+				partial_mfma_body<KT, NA, NB>(tl.bx, tl.by, h, smem);
+			}
 		}
 		__syncthreads();   // the next tile reuses the LDS
 	}
@@ -2436,6 +2485,7 @@ extern "C" void nmfk_plan_half_target(uint32_t p, uint32_t q, uint32_t kt, int p
 		pl->nw = nw;
 		pl->path = (uint32_t)path;
 		pl->lb = lb;
+		pl->ke = kt;
 	};
 	if (lone_fp32 && q >= 1000u && q < 4096u) {
 		fill(2, 2, 12);
@@ -2443,6 +2493,13 @@ extern "C" void nmfk_plan_half_target(uint32_t p, uint32_t q, uint32_t kt, int p
 			return;
 	}
 	fill(na, nb, g_tune[NMFK_TUNE_MIN_STEPS]);
+}
+
+extern "C" void nmfk_plan_set_rank(nmfk_hplan *pl, uint32_t kt, uint32_t k)
+{
+	pl->ke = kt;
+	if (pl->path == NMFK_PATH_MFMA32 && kt == 16u && k <= 8u)
+		pl->ke = k <= 4u ? 4u : 8u;
 }
 
 extern "C" void nmfk_plan_half(uint32_t p, uint32_t q, uint32_t kt, nmfk_hplan *pl)
@@ -2498,6 +2555,22 @@ static int launch_partial_mfma_t(const float *U, size_t ldu, const float *A, con
 	if (pl->bchunk % 16 != 0 || !mfma_shape_ok(na, nb) || (size_t)16 * ldu * 4 > 0x7fffffffull)
 		return (int)hipErrorInvalidValue;
 	const half_args h = {U, ldu, A, B, (int)p, (int)q, (int)pl->bchunk, (int)pl->nab, part, colB, (int)ncolB, den};
+	// A rank up to 4 or 8 at KT 16 sums only its columns in GEMM1 (the default shapes have
+	// these bodies; any other shape sums all 16, which is as correct).
+	if constexpr (KT == 16) {
+		const int ke = (pl->ke == 4 || pl->ke == 8) ? (int)pl->ke : 16;
+#define NMFK_MFMA_KE(NA_, NB_, KE_)                                                                      \
+	if (na == NA_ && nb == NB_ && ke == KE_) {                                                     \
+		hipLaunchKernelGGL((k_partial_mfma<KT, NA_, NB_, KE_>), grid, dim3(64 * NA_ * NB_), 0, stream, h, st, \
+				   (int)pl->S);                                                              \
+		return (int)hipGetLastError();                                                         \
+	}
+		NMFK_MFMA_KE(1, 4, 4)
+		NMFK_MFMA_KE(1, 4, 8)
+		NMFK_MFMA_KE(2, 2, 4)
+		NMFK_MFMA_KE(2, 2, 8)
+#undef NMFK_MFMA_KE
+	}
 #define NMFK_MFMA_SHAPE(NA_, NB_)                                                                        \
 	if (na == NA_ && nb == NB_) {                                                                  \
 		hipLaunchKernelGGL((k_partial_mfma<KT, NA_, NB_>), grid, dim3(64 * NA_ * NB_), 0, stream, h, st, (int)pl->S); \

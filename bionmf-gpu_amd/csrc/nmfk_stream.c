@@ -113,6 +113,10 @@ static void stream_layout(uint32_t n, uint32_t m, uint32_t k, uint32_t block_row
 	nmfk_plan_half_path(m, L->rlast, L->kt, path, &L->bH[1]);
 	nmfk_plan_half_path(br, m, L->kt, path, &L->bW[0]);
 	nmfk_plan_half_path(L->rlast, m, L->kt, path, &L->bW[1]);
+	for (int i = 0; i < 2; i++) {
+		nmfk_plan_set_rank(&L->bH[i], L->kt, k);
+		nmfk_plan_set_rank(&L->bW[i], L->kt, k);
+	}
 	L->SH = (L->nblocks - 1) * L->bH[0].S + L->bH[1].S;
 	const size_t pw0 = (size_t)L->bW[0].S * br, pw1 = (size_t)L->bW[1].S * L->rlast;
 	L->part_cap = (pw0 > pw1 ? pw0 : pw1) * L->kt;

@@ -11,7 +11,7 @@ DEFAULTS = {0: 512, 1: 1024, 2: 8, 3: 64, 4: 1, 5: 4, 6: 256, 7: 2048}
 
 
 class HPlan(C.Structure):
-    _fields_ = [(f, C.c_uint32) for f in ("nab", "S", "bchunk", "ap", "nw", "path", "lb")]
+    _fields_ = [(f, C.c_uint32) for f in ("nab", "S", "bchunk", "ap", "nw", "path", "lb", "ke")]
 
 
 @pytest.fixture
@@ -166,6 +166,22 @@ def test_fp32_short_range_plan(lib):
     assert plan(lib, 20000, 1000, 16).ap == 1
     assert lib.nmfk_tune_set(2, 8) == 0 and lib.nmfk_tune_set(4, 4) == 0 and lib.nmfk_tune_set(5, 1) == 0
     assert (plan(lib, 20000, 1000, 16).ap, plan(lib, 20000, 1000, 16).nw) == (4, 4)
+
+
+def test_plan_rank_width(lib):
+    # FP32 matrix cores at kt 16 sum only a small rank's columns in GEMM1; every other plan
+    # keeps the full padded width.
+    lib.nmfk_plan_set_rank.restype = None
+    lib.nmfk_plan_set_rank.argtypes = [C.POINTER(HPlan), C.c_uint32, C.c_uint32]
+    for k, ke in [(1, 4), (3, 4), (4, 4), (5, 8), (8, 8), (9, 16), (16, 16)]:
+        pl = plan(lib, 1000, 20000, 16)
+        assert pl.ke == 16
+        lib.nmfk_plan_set_rank(C.byref(pl), 16, k)
+        assert pl.ke == ke, k
+    for kt, path, k in [(16, PATH_VALU, 3), (8, PATH_AUTO, 5), (32, PATH_AUTO, 20), (64, PATH_BF16, 40)]:
+        pl = plan(lib, 1000, 20000, kt, path)
+        lib.nmfk_plan_set_rank(C.byref(pl), kt, k)
+        assert pl.ke == kt, (kt, path)
 
 
 @pytest.mark.parametrize("n,m,k,kt", [(20000, 1000, 3, 16), (5000, 38, 3, 16), (5000, 200, 8, 16), (32, 32, 1, 16),

@@ -514,7 +514,7 @@ def main():
         "vs_fp64": accuracy,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": ("k_partial_mfma<16,1,4> (H half) and k_partial_mfma<16,2,2> (W half)"
+                     "kernel": ("k_partial_mfma<16,1,4,16> (H half) and k_partial_mfma<16,2,2,16> (W half)"
                                 if args.kernel_path in ("auto", "mfma32")
                                 else "k_partial<16,...> (vector ALUs, both half-steps)"),
                      "bytes_per_launch": bytes_partial, "ms_per_launch": ms,

@@ -172,8 +172,8 @@ def _integration_blocks():
 
 
 def test_integration_recipe_builds_reference_driver(pkg, tmp_path):
-    """INTEGRATION.md's recipe, word for word: its one-line diff applied to the reference's
-    src/NMF_GPU.c, and its make rule run from a tree laid out like the reference's (the
+    """The integration guide's rebuild recipe, word for word: its one-line diff applied to the
+    reference's src/NMF_GPU.c, and its make rule run from a tree laid out like the reference's (the
     matrix I/O sources and headers are the reference's own, reached through symlinks; only
     the patched driver file is a temporary copy).  It must compile without implicit
     declarations and link against libnmfgpu_amd.so; the built driver's -h runs on the host."""

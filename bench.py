@@ -82,7 +82,7 @@ def cpu_configs(sweep: dict | None) -> dict:
     """The oracle's FP32 scalar rendering of the reference update sequence on the host cores
     for the other configurations (TEST INFRASTRUCTURE, timed as reported baselines only):
     the 5000 x 38 k = 3 factorization in full on 1 and on all threads (the seeds of
-    bench_c0, so the same factorization); the 20000 x 1000 k = 16
+    bench_small_job, so the same factorization); the 20000 x 1000 k = 16
     loop on 1 thread (3 iterations); the rank sweep serially, from each rank's s/iteration
     (20 iterations) times the sweep's own iteration counts; the 50000 x 2000 k = 64 loop
     (2 iterations)."""
@@ -100,14 +100,14 @@ def cpu_configs(sweep: dict | None) -> dict:
         return json.loads(out.strip().splitlines()[-1])
 
     out = {"kind": "port", "threads_all": threads}
-    c0 = {}
+    small = {}
     for th in (1, threads):
         r = run(5000, 38, 3, 2000, th, "lognormal", S=40, mu=5.0, sigma=1.5, seed_v=5, seed_init=7)
-        c0[f"threads_{th}"] = {"factorizations_per_s": 1.0 / r["seconds"], "iterations": r["iters_done"],
-                               "converged": bool(r["converged"])}
-    out["c0_5000x38_k3_full_run"] = c0
+        small[f"threads_{th}"] = {"factorizations_per_s": 1.0 / r["seconds"], "iterations": r["iters_done"],
+                                  "converged": bool(r["converged"])}
+    out["small_5000x38_k3_full_run"] = small
     r = run(N, M, K, 3, 1, "planted", kstar=KSTAR, sigma=SIGMA)
-    out["c1_20000x1000_k16_one_thread"] = {"s_per_iteration": r["s_per_iter"], "sample": "3 iterations"}
+    out["single_large_20000x1000_k16_one_thread"] = {"s_per_iteration": r["s_per_iter"], "sample": "3 iterations"}
     if sweep and "mean_iterations_by_k" in sweep:
         total = 0.0
         per_k = {}
@@ -115,15 +115,15 @@ def cpu_configs(sweep: dict | None) -> dict:
             r = run(5000, 200, int(k_str), 20, threads, "planted", kstar=6, sigma=0.3)
             per_k[k_str] = r["s_per_iter"]
             total += 20 * iters * r["s_per_iter"]
-        out["c2_rank_sweep_serial"] = {"jobs_per_s": 180.0 / total, "s_per_iteration_by_k": per_k,
+        out["rank_sweep_serial"] = {"jobs_per_s": 180.0 / total, "s_per_iteration_by_k": per_k,
                                        "sample": "20 iterations per rank on all threads, times the sweep's "
                                                  "mean iterations per rank and 20 restarts"}
     r = run(50000, 2000, 64, 2, threads, "planted", kstar=48, sigma=0.3)
-    out["c3_50000x2000_k64"] = {"s_per_iteration": r["s_per_iter"], "sample": "2 iterations on all threads"}
+    out["dense_rank_50000x2000_k64"] = {"s_per_iteration": r["s_per_iter"], "sample": "2 iterations on all threads"}
     return out
 
 
-def bench_c0(pkg, runs: int = 5) -> dict:
+def bench_small_job(pkg, runs: int = 5) -> dict:
     """The reference's CPU-runnable case: V 5000 x 38 lognormal(5, 1.5^2), k = 3, stop rule
     T = 10, 40 unchanged tests, 2000 iterations at most; whole factorizations (median of
     <runs>), V resident."""
@@ -471,7 +471,7 @@ def main():
 
     extra = {}
     if not args.no_extra:
-        extra["c0"] = bench_c0(pkg)
+        extra["small_job"] = bench_small_job(pkg)
         extra["sweep"] = bench_sweep(pkg)
         extra["dense_rank"] = bench_dense_rank(pkg)
         extra["served"] = bench_served(pkg, args.served_requests, args.served_rate)

@@ -481,6 +481,18 @@ __device__ __forceinline__ float4 loadq(const float *p)
 // other columns are zero).  Lane group g supplies QN = KE / 4 consecutive columns per 16-column
 // group, column 16h + QN g + qq for K-chunk qq < QN, so a rank up to 8 takes two GEMM1 chunks
 // instead of four and a rank up to 4 one.
+// GEMM2 at KE up to NMFK_G2N_MAX_KE (G2N) runs on the 16-block 4x4x1 FP32 MFMA instead, which
+// does the same multiply-adds per cycle with 4 output columns instead of 16: block b = 4g + jj of lane
+// (g, j = 4jj + n) takes row 4g + r of the 16-row block, rows a = a0 + 16jj + 4i + t (i < 4) and
+// columns c = QN n + cg, so no padding column is computed.  Each lane group g then holds the
+// sums over its own rows of every 16-row block; they are added across the four groups once per
+// split.  Columns KE..KT-1 of the partials are left unwritten: the finish never reads a column
+// at or past the rank.  Only KE 4 takes this: at KE 8 the accumulators double.  The single-job
+// kernel would then run at 140 VGPRs (3 waves per SIMD) and 3.5-5% faster per iteration, but
+// the batched kt-16 classes would spill 160-200 bytes at their 128-register bound (the batched
+// rank sweep slowed by 8%), and a job must give the same bits in a batch as alone, so both
+// keep the 16-wide GEMM2 at KE 8.
+#define NMFK_G2N_MAX_KE 4
 template <int KT, int NA, int NB, int KE = KT>
 __device__ __forceinline__ void partial_mfma_body(int bx, int by, const half_args &ha, float *__restrict__ smem)
 {
@@ -488,6 +500,8 @@ __device__ __forceinline__ void partial_mfma_body(int bx, int by, const half_arg
 	static_assert(KE == KT || (KT == 16 && (KE == 4 || KE == 8)), "GEMM1 width: KT, or 4 / 8 at KT 16");
 	constexpr int H = KT / 16;     // 16-column groups of the rank
 	constexpr int QN = KE >= 16 ? 4 : KE / 4;
+	constexpr bool G2N = KT == 16 && KE <= NMFK_G2N_MAX_KE;
+	constexpr int CW = G2N ? QN : H; // GEMM2 B operands per row and accumulators per t
 	const float *__restrict__ U = ha.U;
 	const size_t ldu = ha.ldu;
 	const float *__restrict__ A = ha.A;
@@ -521,11 +535,11 @@ __device__ __forceinline__ void partial_mfma_body(int bx, int by, const half_arg
 			aop[t][h] = ok ? floatx4{v.x, v.y, v.z, v.w} : floatx4{0.f, 0.f, 0.f, 0.f};
 		}
 	}
-	floatx4 acc[4][H];
+	floatx4 acc[4][CW];
 #pragma unroll
 	for (int t = 0; t < 4; t++)
 #pragma unroll
-		for (int h = 0; h < H; h++)
+		for (int h = 0; h < CW; h++)
 			acc[t][h] = floatx4{0.f, 0.f, 0.f, 0.f};
 
 	const int b_begin = by * bchunk;
@@ -536,13 +550,14 @@ __device__ __forceinline__ void partial_mfma_body(int bx, int by, const half_arg
 	// The buffer descriptors are based at each 16-row block (wave-uniform scalar work), so
 	// byte offsets stay within one block and U and B may exceed the 32-bit range of an offset.
 	const int ucol = avail > 0 ? aj : 0;
+	const int c2 = G2N ? QN * (j & 3) : j; // first GEMM2 output column of this lane
 	int uoff[4], b2off[4][H], b1off[H];
 #pragma unroll
 	for (int r = 0; r < 4; r++) {
 		uoff[r] = ((4 * g + r) * (int)ldu + ucol) * 4;
 #pragma unroll
 		for (int h = 0; h < H; h++)
-			b2off[r][h] = ((4 * g + r) * KT + 16 * h + j) * 4;
+			b2off[r][h] = ((4 * g + r) * KT + 16 * h + c2) * 4;
 	}
 #pragma unroll
 	for (int h = 0; h < H; h++)
@@ -551,11 +566,16 @@ __device__ __forceinline__ void partial_mfma_body(int bx, int by, const half_arg
 	// Everything one 16-row block needs:
 	//   u[r]      U[bb + 4g + r][aj .. aj+3]
 	//   b1[h]     GEMM1 A operand, B[bb + j][16h + QN g + qq], qq < QN
-	//   b2[r][h]  GEMM2 B operand, B[bb + 4g + r][16h + j]
+	//   b2[r][h]  GEMM2 B operand, B[bb + 4g + r][16h + j]; with G2N b2[r][cg] = B[bb + 4g + r][c2 + cg]
 	struct blk {
 		float4 u[4];
 		float4 b1[H];
-		float b2[4][H];
+		float b2[4][CW];
+	};
+	auto load_b2 = [&](blk &x, int r, float4 v) {
+		x.b2[r][0] = v.x;
+		if constexpr (CW > 1)
+			x.b2[r][1] = v.y;
 	};
 	auto load_full = [&](int bb, blk &x) {
 		const __amdgpu_buffer_rsrc_t rU = make_rsrc(U + (size_t)bb * ldu, (size_t)16 * ldu * 4);
@@ -563,9 +583,12 @@ __device__ __forceinline__ void partial_mfma_body(int bx, int by, const half_arg
 #pragma unroll
 		for (int r = 0; r < 4; r++) {
 			x.u[r] = bload4(rU, uoff[r], 0);
+			if constexpr (G2N)
+				load_b2(x, r, bloadq<QN>(rB, b2off[r][0]));
+			else
 #pragma unroll
-			for (int h = 0; h < H; h++)
-				x.b2[r][h] = bload1(rB, b2off[r][h], 0);
+				for (int h = 0; h < H; h++)
+					x.b2[r][h] = bload1(rB, b2off[r][h], 0);
 		}
 #pragma unroll
 		for (int h = 0; h < H; h++)
@@ -578,10 +601,13 @@ __device__ __forceinline__ void partial_mfma_body(int bx, int by, const half_arg
 			const int b = bb + 4 * g + r;
 			const int bs = b < b_end ? b : b_begin;
 			x.u[r] = *reinterpret_cast<const float4 *>(U + (size_t)bs * ldu + ucol);
-			const float *Br = B + (size_t)bs * KT + j;
+			const float *Br = B + (size_t)bs * KT + c2;
+			if constexpr (G2N)
+				load_b2(x, r, loadq<QN>(Br));
+			else
 #pragma unroll
-			for (int h = 0; h < H; h++)
-				x.b2[r][h] = Br[16 * h];
+				for (int h = 0; h < H; h++)
+					x.b2[r][h] = Br[16 * h];
 		}
 		const int b = bb + j;
 		const float *Br = B + (size_t)(b < b_end ? b : b_begin) * KT + QN * g;
@@ -619,6 +645,17 @@ __device__ __forceinline__ void partial_mfma_body(int bx, int by, const half_arg
 		}
 	};
 	auto gemm2 = [&](const blk &x, const float (&rt)[4][4]) {
+		if constexpr (G2N) {
+#pragma unroll
+			for (int r = 0; r < 4; r++)
+#pragma unroll
+				for (int cg = 0; cg < CW; cg++)
+#pragma unroll
+					for (int t = 0; t < 4; t++)
+						acc[t][cg] = __builtin_amdgcn_mfma_f32_4x4x1f32(rt[t][r], x.b2[r][cg], acc[t][cg], 0,
+												0, 0);
+			return;
+		}
 #pragma unroll
 		for (int h = 0; h < H; h++)
 #pragma unroll
@@ -696,6 +733,55 @@ __device__ __forceinline__ void partial_mfma_body(int bx, int by, const half_arg
 		compute(x, bb, std::true_type{});
 	}
 
+	if constexpr (G2N) {
+		// acc[t][cg][i] in lane (g, 4jj + n): rows 4g..4g+3 of each block, a = a0 + 16jj + 4i + t,
+		// c = QN n + cg.  Sum over g, two exchanges (lane groups g ^ 2, then g ^ 1), leaving
+		// component i = g in lane group g: o[t][cg] for a = a0 + 16jj + 4g + t.
+		const bool hi = (g & 2) != 0, odd = (g & 1) != 0;
+		float o[4][CW];
+#pragma unroll
+		for (int t = 0; t < 4; t++)
+#pragma unroll
+			for (int cg = 0; cg < CW; cg++) {
+				const floatx4 v = acc[t][cg];
+				const float k0 = hi ? v[2] : v[0], k1 = hi ? v[3] : v[1];
+				const float s0 = hi ? v[0] : v[2], s1 = hi ? v[1] : v[3];
+				const float e0 = k0 + __shfl_xor(s0, 32), e1 = k1 + __shfl_xor(s1, 32);
+				o[t][cg] = (odd ? e1 : e0) + __shfl_xor(odd ? e0 : e1, 16);
+			}
+		if constexpr (NB > 1) {
+			float (*red)[NA][4][CW][64] = reinterpret_cast<float (*)[NA][4][CW][64]>(smem);
+			if (wb > 0) {
+#pragma unroll
+				for (int t = 0; t < 4; t++)
+#pragma unroll
+					for (int cg = 0; cg < CW; cg++)
+						red[wb - 1][wa][t][cg][lane] = o[t][cg];
+			}
+			__syncthreads();
+			if (wb > 0 || idle)
+				return;
+#pragma unroll
+			for (int w = 0; w < NB - 1; w++)
+#pragma unroll
+				for (int t = 0; t < 4; t++)
+#pragma unroll
+					for (int cg = 0; cg < CW; cg++)
+						o[t][cg] += red[w][wa][t][cg][lane];
+		}
+		if (idle)
+			return;
+		float *dst = ha.part + (size_t)by * p * KT;
+#pragma unroll
+		for (int t = 0; t < 4; t++) {
+			const int a = a0 + 16 * (j >> 2) + 4 * g + t;
+			if (a < p)
+#pragma unroll
+				for (int cg = 0; cg < CW; cg++)
+					dst[(size_t)a * KT + c2 + cg] = o[t][cg];
+		}
+		return;
+	}
 	// acc[t][h][r] = partial for a = a0 + 4*(4g + r) + t, c = 16h + j.
 	if constexpr (NB > 1) {
 		float (*red)[NA][4][H][4][64] = reinterpret_cast<float (*)[NA][4][H][4][64]>(smem);

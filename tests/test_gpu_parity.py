@@ -85,6 +85,36 @@ def test_half_steps_ragged_shapes(nmfk, orc, n, m, k):
         assert rel_err(job.get_W(), W0) < tol
 
 
+@pytest.mark.parametrize("na,nb", [(4, 1), (2, 2), (2, 4), (1, 4)])
+@pytest.mark.parametrize("k", [3, 7])
+def test_small_rank_shapes_match_oracle(nmfk, orc, k, na, nb):
+    # Ranks up to 8 on the FP32 matrix cores: the rank is padded to 16, and both products sum
+    # only the rank's own columns (GEMM2 in 4x4 blocks whose lane groups are added once per
+    # split).  Every workgroup shape gives the FP64 oracle's half-steps: one wave group (no LDS
+    # step, idle waves in the last workgroup at 700 rows) and two or four (LDS step), with
+    # masked last blocks (1300 and 700 rows are not multiples of 16).
+    n, m = 1300, 700
+    old = (nmfk.tune_get(nmfk.TUNE_MFMA_NA), nmfk.tune_get(nmfk.TUNE_MFMA_NB))
+    nmfk.tune_set(nmfk.TUNE_MFMA_NA, na)
+    nmfk.tune_set(nmfk.TUNE_MFMA_NB, nb)
+    try:
+        job, V = make_job(nmfk, orc, n, m, k, seed_v=k + 7, seed_init=k + 8)
+        with job:
+            job.set_kernel_path(2)
+            pl = job.plan()
+            assert pl["H"]["nab"] == -(-m // (64 * na)) and pl["W"]["nab"] == -(-n // (64 * na)), pl
+            for _ in range(2):   # each half-step from the device's own previous factors
+                W, H = job.get_W(), job.get_H()
+                job.update_H()
+                assert rel_err(job.get_H(), orc.update_H(V, W, H)) < 1e-5
+                H = job.get_H()
+                job.update_W()
+                assert rel_err(job.get_W(), orc.update_W(V, W, H)) < 1e-5
+    finally:
+        nmfk.tune_set(nmfk.TUNE_MFMA_NA, old[0])
+        nmfk.tune_set(nmfk.TUNE_MFMA_NB, old[1])
+
+
 def test_golden_tiny_runs(nmfk, orc):
     g = np.load(GOLDEN / "tiny.npz")
     for ci in range(int(g["ncases"])):

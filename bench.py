@@ -16,6 +16,7 @@ contractions) and "served" (a Poisson stream of factorization requests, p50/p99 
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline] [--no-extra]
     python bench.py --served-only --served-requests 2000 --served-rate 4   # the full stream
+    python bench.py --sweep-only   # the batched rank sweep alone
 """
 from __future__ import annotations
 
@@ -171,6 +172,25 @@ def bench_sweep(pkg) -> dict:
                                      for q, k in enumerate(ks)},
             "converged_jobs": int(sum(r["converged"] for r in a["results"])),
             "autotune_seconds": a["tune_seconds"], "tuning": a["tuning"], "plan_target": a["plan_target"]}
+
+
+def bench_sweep_batched(pkg) -> dict:
+    """The rank sweep's batched form alone (the workload of bench_sweep), twice, each with its
+    own autotuning of plan target and launch shape."""
+    n, m = 5000, 200
+    ks, R, base = list(range(2, 11)), 20, 4242
+    src = pkg.Job(n, m, 1)
+    src.gen_V_planted(777, 6, 0.3)
+    a = pkg.sweep(src, ks, R, base, autotune=True, autotune_iters=10)
+    b = pkg.sweep(src, ks, R, base, autotune=True, autotune_iters=10)
+    src.close()
+    nj = len(ks) * R
+    same = all(x["iterations"] == y["iterations"] for x, y in zip(a["results"], b["results"]))
+    return {"workload": f"V {n}x{m} planted rank 6, k=2..10 x {R} restarts = {nj} jobs, T=10, "
+                        "stop_threshold=40, nIters=2000",
+            "batched_jobs_per_s": [nj / a["seconds"], nj / b["seconds"]],
+            "identical_runs": bool(np.array_equal(a["cls"], b["cls"])) and same,
+            "plan_target": a["plan_target"], "tuning": a["tuning"]}
 
 
 def fp64_errors(g, W, H, cls, kl, dv, dvwh) -> dict:
@@ -353,6 +373,9 @@ def main():
                     help="run only one GPU's row block of an 8-way split (dense rank and headline shapes)")
     ap.add_argument("--dense-only", action="store_true",
                     help="run only the dense-rank (k = 64, BF16 vs FP32) workload and print its line")
+    ap.add_argument("--sweep-only", action="store_true",
+                    help="run only the batched rank sweep (twice, autotuned) and print its line; "
+                         "the one-at-a-time comparison stays in the full run")
     ap.add_argument("--kernel-path", choices=sorted(KERNEL_PATHS), default="auto",
                     help="contraction kernels of the single large job (auto: FP32 matrix cores at k = 16); "
                          "valu times the vector-ALU kernels on the same workload")
@@ -375,6 +398,11 @@ def main():
         print(json.dumps({"metric": "dense rank: factorizations/s (BF16 contractions)",
                           "value": d["bf16_factorizations_per_s"], "unit": "factorizations/s", "n_gpus": 1,
                           "higher_is_better": True, "dense_rank": d}))
+        return
+    if args.sweep_only:
+        r = bench_sweep_batched(pkg)
+        print(json.dumps({"metric": "rank sweep, batched: jobs/s", "value": r["batched_jobs_per_s"][-1],
+                          "unit": "jobs/s", "n_gpus": 1, "higher_is_better": True, "sweep": r}))
         return
     if args.size_sweep:
         r = bench_size_sweep(pkg)

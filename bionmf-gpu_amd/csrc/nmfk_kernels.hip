@@ -2540,14 +2540,14 @@ extern "C" void nmfk_plan_half_target(uint32_t p, uint32_t q, uint32_t kt, int p
 	// were changed.  At 50000 x 2000, k = 64 the W half takes 125.5 us against 132.7 us.
 	if (path == NMFK_PATH_MFMA_BF16 && na == 1 && nb == 4 && q < 4096u)
 		na = 2, nb = 2;
-	// Two rules for an FP32 job at kt 16 or 32 planned on its own (no workgroup target, as
+	// Three rules for an FP32 job at kt 16 or 32 planned on its own (no workgroup target, as
 	// batched replans give) with the matrix-core tunables at their defaults.  First, over
 	// 1000-4095 rows of U (the W half of V 20000 x 1000): 2 x 2 waves with at least 12 steps
 	// per wave group, where that still gives 240 workgroups or more.  At 20000 x 1000, k = 16
 	// the W partial takes 18.5-18.7 us in 3 splits against 20.6-21.8 us in 2; shorter ranges
 	// (600 rows) lose 1-2 us, and so do jobs too small to fill the GPU either way (V 2500 x
 	// 1000: 63 workgroups against 85).  Second, the split count is balanced over the CUs
-	// (balanced_splits).
+	// (balanced_splits).  Third, a job with fewer than 240 workgroups takes 4 steps (below).
 	const bool lone_fp32 = path == NMFK_PATH_MFMA32 && kt <= 32u && target_wgs == 0 && na == 1 && nb == 4 &&
 			       g_tune[NMFK_TUNE_MIN_STEPS] == 8 && g_tune[NMFK_TUNE_WGS_MFMA] == 1024 &&
 			       g_tune[NMFK_TUNE_MAX_SPLITS] == 64;
@@ -2599,6 +2599,13 @@ extern "C" void nmfk_plan_half_target(uint32_t p, uint32_t q, uint32_t kt, int p
 			return;
 	}
 	fill(na, nb, g_tune[NMFK_TUNE_MIN_STEPS]);
+	// Third, a lone FP32 job too small for 240 workgroups (the balanced plans' floor) at 8 steps
+	// per wave group takes 4: twice the splits over the idle CUs.  Per iteration: V 5000 x 38,
+	// k = 3 14.8 against 15.8 us; 2500 x 1000, k = 16 20.7-20.9 against 26.4; 3000 x 300,
+	// k = 24 26.7 against 31.8; 1000 x 500, k = 6 18.9 against 22.5; 10000 x 500, k = 8 even.
+	// A plan balanced to 240-255 workgroups (about one per CU) keeps its 8 steps.
+	if (lone_fp32 && (pl->nab + 1u) * pl->S < 240u)
+		fill(na, nb, 4);
 }
 
 extern "C" void nmfk_plan_set_rank(nmfk_hplan *pl, uint32_t kt, uint32_t k)

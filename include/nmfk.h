@@ -158,8 +158,9 @@ int nmfk_job_schedule(const nmfk_job *job);
  * partial kernel aims for (default 256), 7 = waves the one-pass kernel aims for (workgroups x
  * waves per workgroup, default 2048).  With keys 1-5 at their defaults, an FP32 matrix-core
  * job planned on its own (not replanned to a workgroup target) gets 2 x 2 workgroups over a
- * contraction range of 1000-4095 rows where that leaves at least 240 workgroups, and a split
- * count balanced over the CUs (at most the default's); changing any of them turns both off. */
+ * contraction range of 1000-4095 rows where that leaves at least 240 workgroups, a split
+ * count balanced over the CUs (at most the default's), and 4 minimum row steps instead of 8
+ * where 8 leave fewer than 240 workgroups; changing any of them turns all three off. */
 int nmfk_tune_set(int key, int value);
 int nmfk_tune_get(int key);
 

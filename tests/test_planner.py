@@ -147,6 +147,30 @@ def test_balanced_splits(lib, p, q, kt, S):
     assert (pl.nab + 1) * S >= 240 or S == t.S
 
 
+def test_small_lone_job_plan(lib):
+    # A lone FP32 job below 240 workgroups at 8 steps per wave group takes 4.  V 5000 x 38,
+    # k = 3 (padded to 16), H half: 1 workgroup column; 8 steps of 4 x 16 rows cap the 5000
+    # rows at 10 splits (20 workgroups), 4 steps at 20 splits of 256 rows (40 workgroups).
+    h = plan(lib, 38, 5000, 16)
+    assert (h.path, h.nab, h.S, h.bchunk, h.ap) == (PATH_MFMA32, 1, 20, 256, 1)
+    # V 2500 x 1000, k = 16: 85 -> 170 workgroups (H half), 82 -> 164 (W half, 1 x 4)
+    h, w = plan(lib, 1000, 2500, 16), plan(lib, 2500, 1000, 16)
+    assert ((h.nab + 1) * h.S, (w.nab + 1) * w.S, w.ap) == (170, 164, 1)
+    # at 240 workgroups or more the 8 steps stay: V 14000 x 600, k = 12 (H half balanced to
+    # 11 x 23 = 253, W half 220 x 2 = 440 in 304-row splits) and the H half of V 10000 x 1000
+    # (balanced to 17 x 15 = 255)
+    h, w = plan(lib, 600, 14000, 16), plan(lib, 14000, 600, 16)
+    assert ((h.nab + 1) * h.S, (w.nab + 1) * w.S, w.bchunk) == (253, 440, 304)
+    assert plan(lib, 1000, 10000, 16).S == 15
+    # a workgroup target (batched and served plans) or a changed tunable keeps 8 steps:
+    # 10 splits of 512 rows
+    t = HPlan()
+    lib.nmfk_plan_half_target(38, 5000, 16, PATH_AUTO, 16, C.byref(t))
+    assert (t.S, t.bchunk) == (10, 512)
+    assert lib.nmfk_tune_set(1, DEFAULTS[1] // 2) == 0
+    assert plan(lib, 38, 5000, 16).S == 10
+
+
 def test_fp32_short_range_plan(lib):
     # c1's W half (20000 rows of W against the 1000 rows of Vt): 2 x 2 waves, 128 rows of W per
     # workgroup (157 columns), at least 12 steps of 16 rows per wave group, so at most
